@@ -1,0 +1,354 @@
+// C ABI of libgpu_msm.so (declared in include/gpu_msm.h): host orchestration over HIP.
+//
+// Differences from the reference's host side (gpu-crypto/cuda/api.cu:115-160,
+// ic-integration/gpu_crypto/cuda/decompress.cu:191-234), by design:
+//   * one process-wide context per library with grow-only device buffers instead of
+//     five hipMalloc/hipFree pairs per call;
+//   * a private non-blocking stream and a mutex, so concurrent callers are safe;
+//   * every HIP return code is checked; any failure returns -1 (the caller's CPU
+//     fallback) and records a message for blsgpu_last_error();
+//   * per-key line tables can be registered once and stay resident in HBM.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpu_msm.h"
+#include "kernels.hpp"
+#include "verify_engine.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      g_err = std::string(#x) + ": " + hipGetErrorString(e_);                       \
+      return -1;                                                                    \
+    }                                                                               \
+  } while (0)
+#define CHK(x)                                                                      \
+  do {                                                                              \
+    if ((x) != 0) {                                                                 \
+      if (g_err.empty()) g_err = std::string(#x) + " failed";                       \
+      return -1;                                                                    \
+    }                                                                               \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t n) {
+    if (n <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = n + n / 4 + 4096;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      g_err = "hipMalloc failed";
+      p = nullptr;
+      return -1;
+    }
+    cap = want;
+    return 0;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Standard G2 generator in ic_bls12_381's Montgomery form (R = 2^384).
+const uint64_t G2_GEN_REF[24] = {
+    0xf5f28fa202940a10ULL, 0xb3f5fb2687b4961aULL, 0xa1a893b53e2ae580ULL, 0x9894999d1a3caee9ULL,
+    0x6f67b7631863366bULL, 0x058191924350bcd7ULL, 0xa5a9c0759e23f606ULL, 0xaaa0c59dbccd60c3ULL,
+    0x3bb17e18e2867806ULL, 0x1b1ab6cc8541b367ULL, 0xc2b6ed0ef2158547ULL, 0x11922a097360edf3ULL,
+    0x4c730af860494c4aULL, 0x597cfa1f5e369c5aULL, 0xe7e6856caa0a635aULL, 0xbbefb5e96e0d495fULL,
+    0x07d3a975f0ef25a2ULL, 0x0083fd8e7e80dae5ULL, 0xadc0fc92df64b05dULL, 0x18aa270a2b1461dcULL,
+    0x86adac6a3be4eba0ULL, 0x79495c4ec93da33aULL, 0xe7175850a43ccaedULL, 0x0b2bc2a163de1bf2ULL};
+
+struct Ctx {
+  std::mutex mu;
+  int state = 0;  // 0 = not initialized, 1 = ready, -1 = no device
+  int device = 0;
+  hipStream_t s = nullptr;
+  // per-call staging
+  DevBuf sig_ref, nhm_ref, g2_ref, pk_ref, comp, keyidx;
+  DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, fbuf, res, gt;
+  // resident
+  DevBuf gen_lines;  // G2 generator lines (internal), computed at init
+  DevBuf keys;       // registered key tables (internal, key-major)
+  int nkeys = 0;
+  std::vector<uint64_t> host_tmp;
+};
+Ctx g;
+
+int init_locked() {
+  if (g.state == 1) return 0;
+  if (g.state == -1) return -1;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    g.state = -1;
+    g_err = "no HIP device";
+    return -1;
+  }
+  HIPCHK(hipSetDevice(g.device));
+  HIPCHK(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+  // generator lines, once
+  CHK(g.g2_ref.ensure(24 * 8));
+  CHK(g.gen_lines.ensure(bls::KEY_TABLE_WORDS * 4));
+  HIPCHK(hipMemcpyAsync(g.g2_ref.p, G2_GEN_REF, 24 * 8, hipMemcpyHostToDevice, g.s));
+  CHK(bls::launch_g2_prepare(g.g2_ref.as<uint64_t>(), g.gen_lines.as<int32_t>(), 1, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  g.state = 1;
+  return 0;
+}
+
+int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint64_t* g2c,
+                         const uint64_t* pkc, int32_t* results, uint64_t* gt_out, int M) {
+  CHK(init_locked());
+  const size_t m = (size_t)M;
+  CHK(g.sig_ref.ensure(m * 96));
+  CHK(g.nhm_ref.ensure(m * 96));
+  CHK(g.g2_ref.ensure(2448 * 8));
+  CHK(g.pk_ref.ensure(m * 2448 * 8));
+  CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.nhm_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.g2_lines.ensure(bls::KEY_TABLE_WORDS * 4));
+  CHK(g.pk_lines.ensure(m * bls::KEY_TABLE_WORDS * 4));
+  CHK(g.res.ensure(m * 4));
+  CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
+  if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
+  HIPCHK(hipMemcpyAsync(g.sig_ref.p, sigs, m * 96, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.nhm_ref.p, nhms, m * 96, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.g2_ref.p, g2c, 2448 * 8, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.pk_ref.p, pkc, m * 2448 * 8, hipMemcpyHostToDevice, g.s));
+  bls::EngineArgs a{};
+  a.M = M;
+  a.sig_ref = g.sig_ref.as<uint64_t>();
+  a.nhm_ref = g.nhm_ref.as<uint64_t>();
+  a.g2_ref = g.g2_ref.as<uint64_t>();
+  a.pk_ref = g.pk_ref.as<uint64_t>();
+  a.sig_pts = g.sig_pts.as<int32_t>();
+  a.nhm_pts = g.nhm_pts.as<int32_t>();
+  a.g2_lines = g.g2_lines.as<int32_t>();
+  a.pk_lines = g.pk_lines.as<int32_t>();
+  a.fbuf = g.fbuf.as<int32_t>();
+  a.results = g.res.as<int32_t>();
+  a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
+  CHK(bls::engine_legacy(a, g.s));
+  HIPCHK(hipMemcpyAsync(results, g.res.p, m * 4, hipMemcpyDeviceToHost, g.s));
+  if (gt_out) HIPCHK(hipMemcpyAsync(gt_out, g.gt.p, m * 72 * 8, hipMemcpyDeviceToHost, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return 0;
+}
+
+int grow_keys_locked(int add) {
+  size_t need = (size_t)(g.nkeys + add) * bls::KEY_TABLE_WORDS * 4;
+  if (need <= g.keys.cap) return 0;
+  size_t want = need * 2;
+  void* np = nullptr;
+  HIPCHK(hipMalloc(&np, want));
+  if (g.nkeys) HIPCHK(hipMemcpyAsync(np, g.keys.p, (size_t)g.nkeys * bls::KEY_TABLE_WORDS * 4,
+                                      hipMemcpyDeviceToDevice, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  if (g.keys.p) (void)hipFree(g.keys.p);
+  g.keys.p = np;
+  g.keys.cap = want;
+  return 0;
+}
+
+}  // namespace
+
+namespace bls {
+// accessors used by the device-pointer entry points (device_api.cpp)
+hipStream_t lib_stream() { return g.s; }
+int lib_init_locked() { return init_locked(); }
+std::mutex& lib_mutex() { return g.mu; }
+const int32_t* lib_gen_lines() { return g.gen_lines.as<int32_t>(); }
+const int32_t* lib_key_tables() { return g.keys.as<int32_t>(); }
+int lib_num_keys() { return g.nkeys; }
+void lib_set_error(const char* m) { g_err = m; }
+}  // namespace bls
+
+extern "C" {
+
+int gpu_crypto_init(void) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  return init_locked();
+}
+
+int gpu_batch_bls_verify(const uint64_t* sigs, const uint64_t* neg_hms, const uint64_t* g2_coeffs,
+                         const uint64_t* pk_coeffs, int32_t* results, int M) {
+  return blsgpu_batch_verify_gt(sigs, neg_hms, g2_coeffs, pk_coeffs, results, nullptr, M);
+}
+
+int blsgpu_batch_verify_gt(const uint64_t* sigs, const uint64_t* neg_hms, const uint64_t* g2_coeffs,
+                           const uint64_t* pk_coeffs, int32_t* results, uint64_t* gt_out, int M) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (M <= 0 || !sigs || !neg_hms || !g2_coeffs || !pk_coeffs || !results) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  return legacy_verify_locked(sigs, neg_hms, g2_coeffs, pk_coeffs, results, gt_out, M);
+}
+
+int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* status, int n, int check_subgroup) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (n <= 0 || !in48) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  CHK(init_locked());
+  const size_t m = (size_t)n;
+  CHK(g.comp.ensure(m * 48));
+  CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.status.ensure(m * 4));
+  CHK(g.sig_ref.ensure(m * 96));
+  HIPCHK(hipMemcpyAsync(g.comp.p, in48, m * 48, hipMemcpyHostToDevice, g.s));
+  CHK(bls::launch_g1_decompress(g.comp.as<uint8_t>(), g.sig_pts.as<int32_t>(), g.status.as<int32_t>(), n,
+                                check_subgroup, g.s));
+  CHK(bls::launch_points_to_ref(g.sig_pts.as<int32_t>(), g.status.as<int32_t>(), g.sig_ref.as<uint64_t>(), n, g.s));
+  if (out96) HIPCHK(hipMemcpyAsync(out96, g.sig_ref.p, m * 96, hipMemcpyDeviceToHost, g.s));
+  if (status) HIPCHK(hipMemcpyAsync(status, g.status.p, m * 4, hipMemcpyDeviceToHost, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return 0;
+}
+
+int gpu_g1_decompress_batch(const unsigned char* compressed, unsigned char* out_points, int n) {
+  if (n <= 0 || !compressed || !out_points) return -1;
+  std::vector<int32_t> st((size_t)n);
+  if (blsgpu_g1_decompress_status(compressed, out_points, st.data(), n, 1) != 0) return -1;
+  for (int i = 0; i < n; i++)
+    if (st[i] != 0) {
+      g_err = "batch holds a point that is not a valid non-identity G1 element";
+      return -1;
+    }
+  return 0;
+}
+
+int gpu_g1_decompress(const unsigned char* compressed, unsigned char* out_point) {
+  return gpu_g1_decompress_batch(compressed, out_point, 1);
+}
+
+int gpu_msm_g1(const unsigned char*, const unsigned char*, size_t, unsigned char*) {
+  g_err = "G1 MSM is not provided by this library";
+  return -1;
+}
+int gpu_jacobian_to_affine(const unsigned char*, unsigned char*) {
+  g_err = "G1 MSM is not provided by this library";
+  return -1;
+}
+int gpu_batch_sha256(const uint8_t*, uint8_t*, int, int) {
+  g_err = "SHA-256 is not provided by this library";
+  return -1;
+}
+
+int blsgpu_register_key_affine(const uint64_t pk[24]) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (!pk) return -1;
+  CHK(init_locked());
+  CHK(grow_keys_locked(1));
+  CHK(g.g2_ref.ensure(24 * 8));
+  HIPCHK(hipMemcpyAsync(g.g2_ref.p, pk, 24 * 8, hipMemcpyHostToDevice, g.s));
+  CHK(bls::launch_g2_prepare(g.g2_ref.as<uint64_t>(), g.keys.as<int32_t>() + (size_t)g.nkeys * bls::KEY_TABLE_WORDS,
+                             1, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return g.nkeys++;
+}
+
+int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (!coeffs) return -1;
+  CHK(init_locked());
+  CHK(grow_keys_locked(1));
+  CHK(g.pk_ref.ensure(2448 * 8));
+  HIPCHK(hipMemcpyAsync(g.pk_ref.p, coeffs, 2448 * 8, hipMemcpyHostToDevice, g.s));
+  CHK(bls::launch_lines_from_ref(g.pk_ref.as<uint64_t>(),
+                                 g.keys.as<int32_t>() + (size_t)g.nkeys * bls::KEY_TABLE_WORDS, 1, 0, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return g.nkeys++;
+}
+
+int blsgpu_key_coeffs(int handle, uint64_t coeffs[2448]) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (!coeffs || handle < 0 || handle >= g.nkeys) {
+    g_err = "invalid key handle";
+    return -1;
+  }
+  CHK(g.pk_ref.ensure(2448 * 8));
+  CHK(bls::launch_table_to_ref(g.keys.as<int32_t>() + (size_t)handle * bls::KEY_TABLE_WORDS,
+                               g.pk_ref.as<uint64_t>(), g.s));
+  HIPCHK(hipMemcpyAsync(coeffs, g.pk_ref.p, 2448 * 8, hipMemcpyDeviceToHost, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return 0;
+}
+
+int blsgpu_num_keys(void) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  return g.nkeys;
+}
+void blsgpu_reset_keys(void) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g.nkeys = 0;
+}
+
+int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, const int32_t* key_idx,
+                             int32_t* results, uint64_t* gt_out, int M) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (M <= 0 || !sigs48 || !neg_hms || !key_idx || !results) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  CHK(init_locked());
+  for (int i = 0; i < M; i++)
+    if (key_idx[i] < 0 || key_idx[i] >= g.nkeys) {
+      g_err = "key index out of range";
+      return -1;
+    }
+  const size_t m = (size_t)M;
+  CHK(g.comp.ensure(m * 48));
+  CHK(g.nhm_ref.ensure(m * 96));
+  CHK(g.keyidx.ensure(m * 4));
+  CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.nhm_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.status.ensure(m * 4));
+  CHK(g.res.ensure(m * 4));
+  CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
+  if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
+  HIPCHK(hipMemcpyAsync(g.comp.p, sigs48, m * 48, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.nhm_ref.p, neg_hms, m * 96, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.keyidx.p, key_idx, m * 4, hipMemcpyHostToDevice, g.s));
+  bls::EngineArgs a{};
+  a.M = M;
+  a.comp = g.comp.as<uint8_t>();
+  a.nhm_ref = g.nhm_ref.as<uint64_t>();
+  a.key_idx = g.keyidx.as<int32_t>();
+  a.keys = g.keys.as<int32_t>();
+  a.gen_lines = g.gen_lines.as<int32_t>();
+  a.sig_pts = g.sig_pts.as<int32_t>();
+  a.nhm_pts = g.nhm_pts.as<int32_t>();
+  a.status = g.status.as<int32_t>();
+  a.fbuf = g.fbuf.as<int32_t>();
+  a.results = g.res.as<int32_t>();
+  a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
+  CHK(bls::engine_compressed(a, g.s));
+  HIPCHK(hipMemcpyAsync(results, g.res.p, m * 4, hipMemcpyDeviceToHost, g.s));
+  if (gt_out) HIPCHK(hipMemcpyAsync(gt_out, g.gt.p, m * 72 * 8, hipMemcpyDeviceToHost, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return 0;
+}
+
+const char* blsgpu_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+// Curve arithmetic for G1 (y^2 = x^3 + 4 over Fp) and G2 (y^2 = x^3 + 4(1+u) over Fp2):
+// Jacobian group law, G1 decompression with validity checks, the G1 subgroup test and
+// the G2Prepared line coefficients consumed by the Miller loop.
+#pragma once
+#include "pairing.hpp"
+
+namespace bls {
+
+__device__ __forceinline__ bool is_zero(const Fp2& a) { return is_zero(a.c0) && is_zero(a.c1); }
+__device__ __forceinline__ bool eq(const Fp2& a, const Fp2& b) { return is_zero(sub(a, b)); }
+__device__ __forceinline__ Fp2 canon(const Fp2& a) { return {canon(a.c0), canon(a.c1)}; }
+__device__ __forceinline__ Fp sqr_any(const Fp& a) { return sqr(a); }
+__device__ __forceinline__ Fp2 sqr_any(const Fp2& a) { return sqr(a); }
+
+template <class F>
+struct Jac {
+  F x, y, z;
+};
+template <class F>
+struct Aff {
+  F x, y;
+};
+
+// dbl-2009-l (a = 0).  Z = 0 (identity) maps to Z3 = 0.
+template <class F>
+__device__ __forceinline__ Jac<F> jdbl(const Jac<F>& p) {
+  F A = sqr_any(p.x), B = sqr_any(p.y), C = sqr_any(B);
+  F D = norm(dbl(sub(sub(sqr_any(carry(add(p.x, B))), A), C)));
+  F E = norm(add(dbl(A), A));
+  F Fv = sqr_any(E);
+  Jac<F> r;
+  r.x = norm(sub(Fv, dbl(D)));
+  F C8 = norm(dbl(dbl(dbl(C))));  // 8C: normalized before the subtraction (limb range)
+  r.y = norm(sub(mul(E, sub(D, r.x)), C8));
+  r.z = norm(dbl(mul(p.y, p.z)));
+  return r;
+}
+// madd-2007-bl: p Jacobian, q affine (not the identity).
+template <class F>
+__device__ __forceinline__ Jac<F> jadd_aff(const Jac<F>& p, const Aff<F>& q, const F& one) {
+  if (is_zero(p.z)) return {q.x, q.y, one};
+  F Z1Z1 = sqr_any(p.z);
+  F U2 = mul(q.x, Z1Z1);
+  F S2 = mul(q.y, mul(p.z, Z1Z1));
+  F H = norm(sub(U2, p.x));
+  F R = norm(dbl(sub(S2, p.y)));
+  if (is_zero(H)) {
+    if (is_zero(R)) return jdbl(p);
+    Jac<F> o;
+    o.x = one; o.y = one; o.z = norm(sub(one, one));
+    return o;
+  }
+  F HH = sqr_any(H);
+  F I = norm(dbl(dbl(HH)));
+  F J = mul(H, I);
+  F V = mul(p.x, I);
+  Jac<F> r;
+  r.x = norm(sub(sub(sub(sqr_any(R), J), V), V));
+  r.y = norm(sub(mul(R, sub(V, r.x)), dbl(mul(p.y, J))));
+  r.z = norm(sub(sub(sqr_any(carry(add(p.z, H))), Z1Z1), HH));
+  return r;
+}
+template <class F>
+__device__ __forceinline__ bool jeq_aff(const Jac<F>& p, const Aff<F>& q) {
+  if (is_zero(p.z)) return false;
+  F z2 = sqr_any(p.z);
+  return eq(p.x, mul(q.x, z2)) && eq(p.y, mul(q.y, mul(z2, p.z)));
+}
+template <class F>
+__device__ __forceinline__ Aff<F> to_affine(const Jac<F>& p) {
+  F zi = inv(p.z);
+  F zi2 = sqr_any(zi);
+  return {mul(p.x, zi2), mul(p.y, mul(zi2, zi))};
+}
+// scalar (8 x 32-bit words, little-endian) times affine point, MSB first
+template <class F>
+__device__ __forceinline__ Jac<F> smul(const Aff<F>& q, const u32 k[8], const F& one) {
+  Jac<F> acc;
+  acc.x = one; acc.y = one; acc.z = norm(sub(one, one));
+  for (int i = 255; i >= 0; i--) {
+    acc = jdbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1) acc = jadd_aff(acc, q, one);
+  }
+  return acc;
+}
+
+// [|x|] q for the BLS parameter |x| = 0xd201000000010000
+__device__ __forceinline__ Jac<Fp> g1_mul_absx(const Aff<Fp>& q) {
+  Jac<Fp> acc = {q.x, q.y, fp_one()};
+  for (int i = 62; i >= 0; i--) {
+    acc = jdbl(acc);
+    if ((BLS_X >> i) & 1) acc = jadd_aff(acc, q, fp_one());
+  }
+  return acc;
+}
+// G1 membership for a point on the curve: phi(P) = [-x^2]P with phi(x, y) = (beta x, y)
+// (eprint 2021/1130 sec. 6; ic_bls12_381's G1Affine::is_torsion_free).  Equivalently
+// [x^2]P == (beta x, -y).
+__device__ __forceinline__ bool g1_in_subgroup(const Aff<Fp>& p) {
+  Jac<Fp> a = g1_mul_absx(p);
+  if (is_zero(a.z)) return false;
+  Aff<Fp> a1 = to_affine(a);
+  Jac<Fp> b = g1_mul_absx(a1);
+  Aff<Fp> phi = {mul(p.x, fp_const(BETA_M)), neg(p.y)};
+  return jeq_aff(b, phi);
+}
+
+// ---- G1 decompression (ZCash encoding: bit7 compressed, bit6 infinity, bit5 sort) ----
+enum DecodeStatus : int32_t {
+  DEC_OK = 0,
+  DEC_IDENTITY = 1,
+  DEC_BAD_FLAGS = -1,
+  DEC_X_NOT_CANONICAL = -2,
+  DEC_NOT_ON_CURVE = -3,
+  DEC_NOT_IN_SUBGROUP = -4,
+};
+// Decode 48 big-endian bytes given as 12 big-endian-packed 32-bit words (w[0] holds bytes
+// 0..3).  Semantics of ic_bls12_381 G1Affine::from_compressed: the reference GPU kernel
+// (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the flag handling, the
+// x^3+4 square root by (p+1)/4 and the sign choice, but none of the validity checks,
+// which this routine adds.  check_subgroup=false gives from_compressed_unchecked.
+__device__ __forceinline__ int32_t g1_decompress(const u32 be[12], Aff<Fp>& out, bool check_subgroup) {
+  const u32 top = be[0] >> 24;
+  const bool cflag = top & 0x80, iflag = top & 0x40, sflag = top & 0x20;
+  u32 w[12];  // little-endian 32-bit words of x
+#pragma unroll
+  for (int j = 0; j < 12; j++) w[j] = be[11 - j];
+  w[11] &= 0x1fffffffu;
+  out.x = fp_zero();
+  out.y = fp_zero();
+  if (!cflag) return DEC_BAD_FLAGS;
+  if (iflag) {
+    u32 acc = 0;
+#pragma unroll
+    for (int j = 0; j < 12; j++) acc |= w[j];
+    return (!sflag && acc == 0) ? DEC_IDENTITY : DEC_BAD_FLAGS;
+  }
+  // x < p
+  bool lt = false, decided = false;
+#pragma unroll
+  for (int j = 11; j >= 0; j--) {
+    if (!decided && w[j] != P32[j]) {
+      lt = w[j] < P32[j];
+      decided = true;
+    }
+  }
+  if (!lt) return DEC_X_NOT_CANONICAL;
+  Fp x = from_plain(unpack384(w));
+  Fp y2 = norm(add(mul(sqr(x), x), fp_const(B_G1_M)));
+  Fp y = sqrt_cand(y2);
+  if (!eq(sqr(y), y2)) return DEC_NOT_ON_CURVE;
+  if (lex_largest(y) != sflag) y = neg(y);
+  out.x = x;
+  out.y = y;
+  if (check_subgroup && !g1_in_subgroup(out)) return DEC_NOT_IN_SUBGROUP;
+  return DEC_OK;
+}
+// 48-byte compressed encoding of an affine point (big-endian words as above)
+__device__ __forceinline__ void g1_compress(const Aff<Fp>& p, bool identity, u32 be[12]) {
+  if (identity) {
+#pragma unroll
+    for (int j = 0; j < 12; j++) be[j] = 0;
+    be[0] = 0xc0000000u;
+    return;
+  }
+  u32 w[12];
+  pack384(to_plain(p.x), w);
+#pragma unroll
+  for (int j = 0; j < 12; j++) be[j] = w[11 - j];
+  be[0] |= 0x80000000u;
+  if (lex_largest(p.y)) be[0] |= 0x20000000u;
+}
+
+// ---- G2Prepared line coefficients (cuda/test_miller_v2.cu:152-228, eprint 2010/354
+// Alg. 26 and 27).  The projective representation of R is not canonical, so the
+// coefficient values depend on this exact formula sequence; it is the one that
+// reproduces cuda/g2_coeffs.h and cuda/pk42_coeffs.h. ----
+struct G2Proj {
+  Fp2 x, y, z;
+};
+__device__ __forceinline__ Line doubling_step(G2Proj& r) {
+  Fp2 tmp0 = sqr(r.x);
+  Fp2 tmp1 = sqr(r.y);
+  Fp2 tmp2 = sqr(tmp1);
+  Fp2 tmp3 = norm(dbl(sub(sqr(carry(add(tmp1, r.x))), add(tmp0, tmp2))));
+  Fp2 tmp4 = norm(add(dbl(tmp0), tmp0));
+  Fp2 tmp6 = carry(add(r.x, tmp4));
+  Fp2 tmp5 = sqr(tmp4);
+  Fp2 zsq = sqr(r.z);
+  Fp2 nx = norm(sub(sub(tmp5, tmp3), tmp3));
+  Fp2 nz = norm(sub(sqr(carry(add(r.z, r.y))), add(tmp1, zsq)));
+  Fp2 ny = norm(sub(mul(sub(tmp3, nx), tmp4), norm(dbl(dbl(dbl(tmp2))))));
+  r.x = nx; r.y = ny; r.z = nz;
+  Fp2 c1 = norm(neg(dbl(mul(tmp4, zsq))));
+  Fp2 c2 = norm(sub(sub(sqr(tmp6), add(tmp0, tmp5)), dbl(dbl(tmp1))));
+  Fp2 c0 = norm(dbl(mul(r.z, zsq)));
+  return {c0, c1, c2};
+}
+__device__ __forceinline__ Line addition_step(G2Proj& r, const Aff<Fp2>& q) {
+  Fp2 zsq = sqr(r.z);
+  Fp2 ysq = sqr(q.y);
+  Fp2 t0 = mul(zsq, q.x);
+  Fp2 t1 = mul(norm(sub(sqr(carry(add(q.y, r.z))), add(ysq, zsq))), zsq);
+  Fp2 t2 = norm(sub(t0, r.x));
+  Fp2 t3 = sqr(t2);
+  Fp2 t4 = norm(dbl(dbl(t3)));
+  Fp2 t5 = mul(t4, t2);
+  Fp2 t6 = norm(sub(sub(t1, r.y), r.y));
+  Fp2 t9 = mul(t6, q.x);
+  Fp2 t7 = mul(t4, r.x);
+  Fp2 nx = norm(sub(sub(sub(sqr(t6), t5), t7), t7));
+  Fp2 nz = norm(sub(sqr(carry(add(r.z, t2))), add(zsq, t3)));
+  Fp2 t10 = carry(add(q.y, nz));
+  Fp2 t8 = mul(sub(t7, nx), t6);
+  Fp2 ny = norm(sub(t8, dbl(mul(r.y, t5))));
+  r.x = nx; r.y = ny; r.z = nz;
+  t10 = norm(sub(sub(sqr(t10), ysq), sqr(nz)));
+  Fp2 c2 = norm(sub(dbl(t9), t10));
+  Fp2 c0 = norm(dbl(nz));
+  Fp2 c1 = norm(dbl(neg(t6)));
+  return {c0, c1, c2};
+}
+// Emit the 68 lines of q in Miller-loop order through sink(k, line).
+template <class Sink>
+__device__ __forceinline__ void g2_prepare(const Aff<Fp2>& q, Sink sink) {
+  G2Proj r = {q.x, q.y, fp2_one()};
+  int n = 0;
+  for (int b = 61; b >= 0; b--) {
+    sink(n++, doubling_step(r));
+    if (miller_bit(b)) sink(n++, addition_step(r, q));
+  }
+  sink(n++, doubling_step(r));
+}
+
+}  // namespace bls

@@ -1,0 +1,272 @@
+// Batch BLS verification kernels for gfx950 (one thread per item).
+//
+// Device data formats (all little-endian):
+//   internal Fp  : 14 x i32 limbs (field.hpp), value congruent to x * 2^392 mod p
+//   G1 points    : SoA over items, word w (0..27 = x limbs, y limbs) of item i at
+//                  pts[w * M + i]  (coalesced across a wavefront)
+//   line tables  : one key = 68 lines x (c0, c1, c2 in Fp2) = 68 x 84 words; per-key
+//                  tables are stored key-major (AoS, read with 16-byte loads), per-item
+//                  tables of the legacy ABI are stored SoA: word w of line k of item i
+//                  at lines[(k * 84 + w) * M + i]
+//   Miller value : SoA, 168 words (12 Fp) per item
+#include <hip/hip_runtime.h>
+
+#include "curve.hpp"
+#include "kernels.hpp"
+
+namespace bls {
+
+constexpr int LINE_WORDS = 84;
+constexpr int KEY_WORDS = NUM_COEFFS * LINE_WORDS;  // 5712
+constexpr int PT_WORDS = 2 * NL;                    // 28
+constexpr int F12_WORDS = 12 * NL;                  // 168
+
+__device__ __forceinline__ Fp ld_fp_soa(const i32* base, int w0, int M, int i) {
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < NL; j++) r.l[j] = base[(size_t)(w0 + j) * M + i];
+  return r;
+}
+__device__ __forceinline__ void st_fp_soa(i32* base, int w0, int M, int i, const Fp& v) {
+#pragma unroll
+  for (int j = 0; j < NL; j++) base[(size_t)(w0 + j) * M + i] = v.l[j];
+}
+__device__ __forceinline__ Fp ld_fp_aos(const i32* p) {
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < NL; j++) r.l[j] = p[j];
+  return r;
+}
+__device__ __forceinline__ void st_fp_aos(i32* p, const Fp& v) {
+#pragma unroll
+  for (int j = 0; j < NL; j++) p[j] = v.l[j];
+}
+__device__ __forceinline__ Fp ld_ref(const u64* p) {
+  u32 w[12];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    u64 v = p[j];
+    w[2 * j] = (u32)v;
+    w[2 * j + 1] = (u32)(v >> 32);
+  }
+  return from_ref(w);
+}
+__device__ __forceinline__ void st_ref(u64* p, const Fp& a) {
+  u32 w[12];
+  to_ref(a, w);
+#pragma unroll
+  for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
+}
+
+// ---- line sources for the Miller loop ----
+struct LinesAoS {  // one key table (same for every lane or chosen per lane)
+  const i32* t;
+  __device__ __forceinline__ Line operator()(int k) const {
+    const int4* q = reinterpret_cast<const int4*>(t + k * LINE_WORDS);
+    i32 w[LINE_WORDS];
+#pragma unroll
+    for (int j = 0; j < LINE_WORDS / 4; j++) {
+      int4 v = q[j];
+      w[4 * j] = v.x; w[4 * j + 1] = v.y; w[4 * j + 2] = v.z; w[4 * j + 3] = v.w;
+    }
+    Line L;
+    Fp* f = &L.c0.c0;
+#pragma unroll
+    for (int e = 0; e < 6; e++)
+#pragma unroll
+      for (int j = 0; j < NL; j++) f[e].l[j] = w[e * NL + j];
+    return L;
+  }
+};
+struct LinesSoA {  // per-item tables of the legacy ABI
+  const i32* t;
+  int M, i;
+  __device__ __forceinline__ Line operator()(int k) const {
+    Line L;
+    Fp* f = &L.c0.c0;
+#pragma unroll
+    for (int e = 0; e < 6; e++) f[e] = ld_fp_soa(t, k * LINE_WORDS + e * NL, M, i);
+    return L;
+  }
+};
+
+// ---- kernels ----
+
+// legacy ABI points: M x 12 u64 (x | y, reference Montgomery form) -> internal SoA
+__global__ void k_points_from_ref(const u64* __restrict__ in, i32* __restrict__ out, int M) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  st_fp_soa(out, 0, M, i, ld_ref(in + (size_t)i * 12));
+  st_fp_soa(out, NL, M, i, ld_ref(in + (size_t)i * 12 + 6));
+}
+
+// legacy ABI line tables: M x 2448 u64 (68 x 6 Fp, reference form) -> internal.
+// One thread per (item, Fp); soa != 0 writes the per-item SoA layout, else key-major AoS.
+__global__ void k_lines_from_ref(const u64* __restrict__ in, i32* __restrict__ out, int M, int soa) {
+  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = NUM_COEFFS * 6;  // 408 Fp per key
+  if (g >= (long long)M * per) return;
+  int i = (int)(g / per), f = (int)(g % per);
+  Fp v = ld_ref(in + (size_t)i * (per * 6) + (size_t)f * 6);
+  if (soa) st_fp_soa(out, f * NL, M, i, v);
+  else st_fp_aos(out + (size_t)i * KEY_WORDS + f * NL, v);
+}
+
+// G2Prepared of K affine points given in reference form (24 u64: x.c0 x.c1 y.c0 y.c1)
+// -> key-major internal tables (K x 5712 words)
+__global__ void k_g2_prepare(const u64* __restrict__ q, i32* __restrict__ out, int K) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  const u64* s = q + (size_t)i * 24;
+  Aff<Fp2> Q = {{ld_ref(s), ld_ref(s + 6)}, {ld_ref(s + 12), ld_ref(s + 18)}};
+  i32* dst = out + (size_t)i * KEY_WORDS;
+  g2_prepare(Q, [&](int k, const Line& L) {
+    const Fp* f = &L.c0.c0;
+#pragma unroll
+    for (int e = 0; e < 6; e++) st_fp_aos(dst + k * LINE_WORDS + e * NL, f[e]);
+  });
+}
+
+// compressed G1 (M x 48 bytes) -> internal SoA points + status per item
+__global__ void k_g1_decompress(const uint8_t* __restrict__ in, i32* __restrict__ pts, i32* __restrict__ status,
+                                int M, int check_subgroup) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  u32 be[12];
+  const uint8_t* b = in + (size_t)i * 48;
+#pragma unroll
+  for (int j = 0; j < 12; j++)
+    be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
+  Aff<Fp> P;
+  int32_t st = g1_decompress(be, P, check_subgroup != 0);
+  st_fp_soa(pts, 0, M, i, P.x);
+  st_fp_soa(pts, NL, M, i, P.y);
+  status[i] = st;
+}
+
+// internal SoA points -> reference form x | y (M x 12 u64); identity status -> zeros
+__global__ void k_points_to_ref(const i32* __restrict__ pts, const i32* __restrict__ status, u64* __restrict__ out,
+                                int M) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  u64* o = out + (size_t)i * 12;
+  if (status && status[i] != DEC_OK) {
+#pragma unroll
+    for (int j = 0; j < 12; j++) o[j] = 0;
+    return;
+  }
+  st_ref(o, ld_fp_soa(pts, 0, M, i));
+  st_ref(o + 6, ld_fp_soa(pts, NL, M, i));
+}
+
+// two-pair Miller loop.  Pair 1: (sig, G2 generator lines); pair 2: (-H(m), pk lines).
+// pk lines come from per-item SoA tables (key_idx == nullptr) or from key-major tables
+// selected by key_idx[i].  sig_status (nullable): DEC_IDENTITY drops pair 1, any other
+// non-OK status skips the item (its verdict is set by the final kernel).
+__global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nhm, const i32* __restrict__ g2lines,
+                         const i32* __restrict__ pklines, const int32_t* __restrict__ key_idx,
+                         const i32* __restrict__ sig_status, i32* __restrict__ fout, int M) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  bool skip1 = false;
+  if (sig_status) {
+    int32_t st = sig_status[i];
+    if (st != DEC_OK && st != DEC_IDENTITY) return;
+    skip1 = st == DEC_IDENTITY;
+  }
+  G1Aff p1 = {ld_fp_soa(sig, 0, M, i), ld_fp_soa(sig, NL, M, i)};
+  G1Aff p2 = {ld_fp_soa(nhm, 0, M, i), ld_fp_soa(nhm, NL, M, i)};
+  Fp12 f;
+  LinesAoS gen{g2lines};
+  if (key_idx) {
+    LinesAoS pk{pklines + (size_t)key_idx[i] * KEY_WORDS};
+    f = miller2(p1, gen, p2, pk, skip1, false);
+  } else {
+    LinesSoA pk{pklines, M, i};
+    f = miller2(p1, gen, p2, pk, skip1, false);
+  }
+  const Fp* v = &f.c0.c0.c0;
+#pragma unroll
+  for (int e = 0; e < 12; e++) st_fp_soa(fout, e * NL, M, i, v[e]);
+}
+
+// final exponentiation + identity test.  result[i] = 1 valid, 0 invalid, or the
+// (negative) decode status of the signature when sig_status is given.  gt_out
+// (nullable): M x 72 u64 in reference form (ic_bls12_381 Gt layout).
+__global__ void k_final_exp(const i32* __restrict__ fin, const i32* __restrict__ sig_status,
+                            int32_t* __restrict__ result, u64* __restrict__ gt_out, int M) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  if (sig_status) {
+    int32_t st = sig_status[i];
+    if (st != DEC_OK && st != DEC_IDENTITY) {
+      result[i] = st;
+      if (gt_out) {
+        for (int j = 0; j < 72; j++) gt_out[(size_t)i * 72 + j] = 0;
+      }
+      return;
+    }
+  }
+  Fp12 f;
+  Fp* v = &f.c0.c0.c0;
+#pragma unroll
+  for (int e = 0; e < 12; e++) v[e] = ld_fp_soa(fin, e * NL, M, i);
+  Fp12 r = final_exp_hard(final_exp_easy(f));
+  result[i] = is_one(r) ? 1 : 0;
+  if (gt_out) {
+    const Fp* rv = &r.c0.c0.c0;
+#pragma unroll
+    for (int e = 0; e < 12; e++) st_ref(gt_out + (size_t)i * 72 + e * 6, rv[e]);
+  }
+}
+
+// internal key table -> reference form (2448 u64), one thread per Fp
+__global__ void k_table_to_ref(const i32* __restrict__ t, u64* __restrict__ out) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= NUM_COEFFS * 6) return;
+  st_ref(out + (size_t)f * 6, ld_fp_aos(t + f * NL));
+}
+
+// ---- launch wrappers (host) ----
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s) {
+  hipLaunchKernelGGL(k_points_from_ref, dim3(nblk(M, 64)), dim3(64), 0, s, in, out, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, int soa, hipStream_t s) {
+  long long n = (long long)M * NUM_COEFFS * 6;
+  hipLaunchKernelGGL(k_lines_from_ref, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, M, soa);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_table_to_ref, dim3(nblk(NUM_COEFFS * 6, 64)), dim3(64), 0, s, t, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_g2_prepare(const uint64_t* q, int32_t* out, int K, hipStream_t s) {
+  hipLaunchKernelGGL(k_g2_prepare, dim3(nblk(K, 64)), dim3(64), 0, s, q, out, K);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_g1_decompress(const uint8_t* in, int32_t* pts, int32_t* status, int M, int check_subgroup,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_g1_decompress, dim3(nblk(M, 64)), dim3(64), 0, s, in, pts, status, M, check_subgroup);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* out, int M, hipStream_t s) {
+  hipLaunchKernelGGL(k_points_to_ref, dim3(nblk(M, 64)), dim3(64), 0, s, pts, status, out, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines, const int32_t* pklines,
+                  const int32_t* key_idx, const int32_t* sig_status, int32_t* fout, int M, int block,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_miller, dim3(nblk(M, block)), dim3(block), 0, s, sig, nhm, g2lines, pklines, key_idx,
+                     sig_status, fout, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
+                     int block, hipStream_t s) {
+  hipLaunchKernelGGL(k_final_exp, dim3(nblk(M, block)), dim3(block), 0, s, fin, sig_status, result, gt_out, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace bls

@@ -1,0 +1,26 @@
+// Host-side launch wrappers of the verification kernels (kernels.hip).
+// All pointers are device pointers; every wrapper returns 0 or -1 (launch error).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bls {
+
+constexpr int KEY_TABLE_WORDS = 68 * 84;  // internal G2Prepared table of one key
+constexpr int POINT_WORDS = 28;           // internal affine G1 point
+constexpr int MILLER_WORDS = 168;         // internal Fp12
+
+int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s);
+int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, int soa, hipStream_t s);
+int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s);
+int launch_g2_prepare(const uint64_t* q, int32_t* out, int K, hipStream_t s);
+int launch_g1_decompress(const uint8_t* in, int32_t* pts, int32_t* status, int M, int check_subgroup,
+                         hipStream_t s);
+int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* out, int M, hipStream_t s);
+int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines, const int32_t* pklines,
+                  const int32_t* key_idx, const int32_t* sig_status, int32_t* fout, int M, int block,
+                  hipStream_t s);
+int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
+                     int block, hipStream_t s);
+
+}  // namespace bls

@@ -1,0 +1,37 @@
+// Kernel sequences for one verification batch.  Buffers are supplied by the caller
+// (the library context or a device-resident caller); the engine allocates nothing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bls {
+
+struct EngineArgs {
+  int M = 0;
+  // inputs (device pointers)
+  const uint64_t* sig_ref = nullptr;  // legacy: M x 12 u64 uncompressed signatures
+  const uint8_t* comp = nullptr;      // fused: M x 48 compressed signatures
+  const uint64_t* nhm_ref = nullptr;  // M x 12 u64, -H(m)
+  const uint64_t* g2_ref = nullptr;   // legacy: generator lines, 2448 u64
+  const uint64_t* pk_ref = nullptr;   // legacy: M x 2448 u64 per-item lines
+  const int32_t* key_idx = nullptr;   // fused: M key handles
+  const int32_t* keys = nullptr;      // fused: resident key tables (internal)
+  const int32_t* gen_lines = nullptr; // fused: generator table (internal)
+  // workspace
+  int32_t* sig_pts = nullptr;   // M x 28
+  int32_t* nhm_pts = nullptr;   // M x 28
+  int32_t* g2_lines = nullptr;  // 5712 (legacy)
+  int32_t* pk_lines = nullptr;  // M x 5712 (legacy)
+  int32_t* status = nullptr;    // M (fused)
+  int32_t* fbuf = nullptr;      // M x 168 Miller values
+  // outputs
+  int32_t* results = nullptr;  // M
+  uint64_t* gt = nullptr;      // M x 72 (nullable)
+  // tuning
+  int block = 64;
+};
+
+int engine_legacy(const EngineArgs& a, hipStream_t s);
+int engine_compressed(const EngineArgs& a, hipStream_t s);
+
+}  // namespace bls

@@ -1,0 +1,105 @@
+/*
+ * libgpu_msm.so -- MI355X (gfx950) batch BLS12-381 signature verification.
+ *
+ * Drop-in for the reference's GPU crypto ABI.  The first block of entry points
+ * keeps the exact names, argument meaning, data layout and return convention of
+ * the reference (all pointers are caller-owned host memory, calls are
+ * synchronous, 0 = success, -1 = failure, on which callers fall back to the CPU).
+ * Unlike the reference, every entry point is thread-safe.
+ *
+ * Data layout (ic_bls12_381 internal form, as marshalled by the reference's Rust
+ * crate, ic-integration/gpu_crypto/lib.rs:45-69):
+ *   Fp        6 x u64 little-endian, Montgomery form with R = 2^384, reduced to [0, p)
+ *   G1 point  x | y = 12 x u64 (no infinity flag)
+ *   G2 point  x.c0 | x.c1 | y.c0 | y.c1 = 24 x u64
+ *   line      (c0, c1, c2) in Fp2^3 = 36 x u64; one key = 68 lines = 2448 x u64
+ *   GT        12 Fp = 72 x u64 in the order c0.c0.c0, c0.c0.c1, c0.c1.c0, ..., c1.c2.c1
+ */
+#ifndef GPU_MSM_H
+#define GPU_MSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================== reference ABI ===================== */
+
+/* Replaces gpu-crypto/cuda/api.cu:118-124.  0 when a GPU is present, -1 otherwise. */
+int gpu_crypto_init(void);
+
+/* Replaces gpu-crypto/cuda/api.cu:128-160 (Rust binding ic-integration/gpu_crypto/lib.rs:20-27).
+ * Verifies e(sigs[i], G2) * e(neg_hms[i], pk_i) == 1 for i < M, with pk_i given by its
+ * 68 precomputed lines pk_coeffs + i*2448 and the generator's lines in g2_coeffs.
+ * results[i] = 1 (valid) or 0 (invalid).  Returns 0, or -1 on any device error or
+ * M <= 0 (the reference's M = 0 launch also fails). */
+int gpu_batch_bls_verify(const uint64_t* sigs, const uint64_t* neg_hms, const uint64_t* g2_coeffs,
+                         const uint64_t* pk_coeffs, int32_t* results, int M);
+
+/* Replaces ic-integration/gpu_crypto/cuda/decompress.cu:198-226.  n x 48-byte ZCash-
+ * compressed G1 points -> n x 96 bytes (x | y, ic_bls12_381 internal form).  Returns -1
+ * when n <= 0 or when ANY point is not a valid non-identity G1 element (bad flags,
+ * x >= p, x^3+4 not a square, not in the r-order subgroup, or the identity encoding):
+ * the caller then decodes the batch on the CPU and gets ic_bls12_381's exact error. */
+int gpu_g1_decompress_batch(const unsigned char* compressed, unsigned char* out_points, int n);
+
+/* Replaces decompress.cu:229-231: gpu_g1_decompress_batch with n = 1. */
+int gpu_g1_decompress(const unsigned char* compressed, unsigned char* out_point);
+
+/* Replace ic-integration/gpu_crypto/cuda/msm.cu:32-56.  G1 MSM is not part of this
+ * library; both symbols exist because the caller's dlopen requires all three of
+ * gpu_msm_g1, gpu_jacobian_to_affine and gpu_g1_decompress_batch
+ * (ic-integration/gpu_crypto/lib.rs:104-116).  Always -1 (caller uses the CPU). */
+int gpu_msm_g1(const unsigned char* points, const unsigned char* scalars, size_t n, unsigned char* out);
+int gpu_jacobian_to_affine(const unsigned char* jac, unsigned char* aff);
+
+/* Replaces gpu-crypto/cuda/api.cu:165-186 (linked by lib.rs:14-19).  Out of scope:
+ * always -1, the precedent being the reference's own stub build (BUILD.bazel:38). */
+int gpu_batch_sha256(const uint8_t* chunks, uint8_t* hashes, int n, int chunk_size);
+
+/* ===================== extensions (no reference counterpart) ===================== */
+
+/* Per-item codes of the extended verify entry points. */
+#define BLSGPU_VALID 1
+#define BLSGPU_INVALID 0
+#define BLSGPU_ERR_BAD_FLAGS (-1)       /* signature encoding flags */
+#define BLSGPU_ERR_X_NOT_CANONICAL (-2) /* signature x >= p */
+#define BLSGPU_ERR_NOT_ON_CURVE (-3)    /* x^3 + 4 has no square root */
+#define BLSGPU_ERR_NOT_IN_SUBGROUP (-4) /* point outside the r-order subgroup */
+
+/* gpu_batch_bls_verify plus the GT element of every item (M x 72 u64, nullable). */
+int blsgpu_batch_verify_gt(const uint64_t* sigs, const uint64_t* neg_hms, const uint64_t* g2_coeffs,
+                           const uint64_t* pk_coeffs, int32_t* results, uint64_t* gt_out, int M);
+
+/* Key registry: per-key line tables stay resident in HBM.  Returns a handle >= 0 or -1.
+ * from_affine computes G2Prepared on the GPU from an affine G2 point (24 u64);
+ * from_coeffs takes the 68 precomputed lines (2448 u64). */
+int blsgpu_register_key_affine(const uint64_t pk[24]);
+int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]);
+/* copies a registered key's lines back out (2448 u64, reference form) */
+int blsgpu_key_coeffs(int handle, uint64_t coeffs[2448]);
+int blsgpu_num_keys(void);
+void blsgpu_reset_keys(void);
+
+/* Fused path: M compressed signatures (M x 48 bytes) are decompressed and validated on
+ * the device and verified against e(.,G2) * e(neg_hms[i], key[key_idx[i]]).
+ * results[i]: BLSGPU_VALID, BLSGPU_INVALID or a negative BLSGPU_ERR_* decode code.
+ * An identity signature is decoded and paired as the identity (ic_bls12_381 semantics).
+ * gt_out nullable (zeros for items with a decode error). */
+int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, const int32_t* key_idx,
+                             int32_t* results, uint64_t* gt_out, int M);
+
+/* Decompress with per-point status (n x 96 bytes out; status[i] as BLSGPU_ERR_*, 0 ok,
+ * 1 identity).  check_subgroup = 0 skips the subgroup test. */
+int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* status, int n,
+                                int check_subgroup);
+
+/* Last error message of the calling thread ("" if none). */
+const char* blsgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

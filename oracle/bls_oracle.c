@@ -723,6 +723,30 @@ void orc_miller2(const u64 p1[12], const u64 q1c[2448], const u64 p2[12], const 
     miller2(&r, &a, (const fp2(*)[3])q1c, &b, (const fp2(*)[3])q2c);
     ST(out, r);
 }
+/* one-pair Miller loop over prepared lines (the two-pair loop with a single term) */
+void orc_miller1(const u64 p1[12], const u64 q1c[2448], u64 out[72]) {
+    LD(g1a, a, p1);
+    const fp2(*q)[3] = (const fp2(*)[3])q1c;
+    fp12 f = fp12_one();
+    int ci = 0, found = 0;
+    for (int b = 63; b >= 0; b--) {
+        int bit = (int)(((BLS_X >> 1) >> b) & 1);
+        if (!found) { found = bit; continue; }
+        ell(&f, q[ci++], &a);
+        if (bit) ell(&f, q[ci++], &a);
+        fp12_sqr(&f, &f);
+    }
+    ell(&f, q[ci], &a);
+    fp12 r; fp12_conj(&r, &f);
+    ST(out, r);
+}
+/* e(P, Q) for affine P (12 u64) and affine Q (24 u64) */
+void orc_pairing(const u64 p[12], const u64 q[24], u64 out[72]) {
+    u64 c[2448], ml[72];
+    orc_g2_prepare(q, c);
+    orc_miller1(p, c, ml);
+    orc_final_exp(ml, out);
+}
 void orc_final_exp(const u64 f[72], u64 out[72]) { LD(fp12, x, f); fp12 r; final_exp(&r, &x); ST(out, r); }
 int orc_verify(const u64 sig[12], const u64 neg_hm[12], const u64 g2c[2448], const u64 pkc[2448], u64 gt_out[72]) {
     LD(g1a, s, sig); LD(g1a, n, neg_hm);

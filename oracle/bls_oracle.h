@@ -67,6 +67,8 @@ void orc_g2_prepare(const uint64_t q[24], uint64_t coeffs[2448]);
 void orc_miller2(const uint64_t p1[12], const uint64_t q1c[2448],
                  const uint64_t p2[12], const uint64_t q2c[2448], uint64_t out[72]);
 void orc_final_exp(const uint64_t f[72], uint64_t out[72]);
+void orc_miller1(const uint64_t p1[12], const uint64_t q1c[2448], uint64_t out[72]);
+void orc_pairing(const uint64_t p[12], const uint64_t q[24], uint64_t out[72]);
 /* e(sig,G2)*e(neg_hm,pk) == 1 ; gt_out (nullable) receives the GT element. */
 int  orc_verify(const uint64_t sig[12], const uint64_t neg_hm[12], const uint64_t g2c[2448],
                 const uint64_t pkc[2448], uint64_t gt_out[72]);

@@ -177,8 +177,9 @@ def main():
         "fp2_mul_nr": {"a": [3, 7], "out": [-4, 10]},
         "fp6_mul": {"a": [1, 2, 3, 4, 5, 6], "b": [7, 8, 9, 10, 11, 12],
                     "out": [-223, 176, -165, 189, -39, 182]},
+        # Fp2 components c0.c0, c0.c1 and c1.c0 of the product (tower index 0, 1, 3)
         "fp12_mul": {"a": list(range(1, 13)), "b": list(range(13, 25)),
-                     "out_first6": [-1650, 1405, -1282, 1475, -1238, 1240]},
+                     "out_fp2": {"0": [-1650, 1405], "1": [-1282, 1475], "3": [-1238, 1240]}},
     }
 
     fx["_sources"] = __doc__.split("Sources (paths relative to the reference root):")[1].strip()

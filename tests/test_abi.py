@@ -1,0 +1,63 @@
+"""The drop-in boundary on the CPU: libgpu_msm.so loads, exports every symbol that
+include/gpu_msm.h declares, and keeps the reference's failure conventions.  No
+compute entry point is called here (there is no GPU in this environment)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import blsgpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gpu_msm.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpu_\w+|blsgpu_\w+)\s*\(", text)))
+
+
+def test_header_declares_reference_abi():
+    syms = declared_symbols()
+    for s in blsgpu.REFERENCE_SYMBOLS:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = blsgpu.load()
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", blsgpu.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if l.strip())
+    for s in declared_symbols():
+        assert s in exported, s
+
+
+def test_dlopen_three_symbol_check_like_the_replica():
+    # ic-integration/gpu_crypto/lib.rs:92-122: the library is used only if all three resolve
+    h = ctypes.CDLL(blsgpu.LIB_PATH, mode=os.RTLD_NOW | os.RTLD_GLOBAL)
+    for s in ("gpu_msm_g1", "gpu_jacobian_to_affine", "gpu_g1_decompress_batch"):
+        assert getattr(h, s) is not None
+
+
+def test_out_of_scope_entry_points_fail_so_callers_use_cpu():
+    lib = blsgpu.load()
+    assert lib.gpu_msm_g1(None, None, 0, None) == -1
+    assert lib.gpu_jacobian_to_affine(None, None) == -1
+    assert lib.gpu_batch_sha256(None, None, 0, 0) == -1
+
+
+def test_invalid_sizes_rejected_before_any_device_work():
+    lib = blsgpu.load()
+    assert lib.gpu_batch_bls_verify(None, None, None, None, None, 0) == -1
+    assert lib.gpu_g1_decompress_batch(None, None, 0) == -1
+    assert lib.gpu_g1_decompress_batch(None, None, -3) == -1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_init_reports_no_device():
+    # gpu-crypto/cuda/api.cu:118-124: -1 when no device is found
+    assert blsgpu.init() == -1
