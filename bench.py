@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Benchmark: batch BLS12-381 signature verification on one MI355X.
+
+A step is one pass of the verification path over one batch: the plain GPU form on a
+uniform batch of 4096 verifications with 4096 distinct keys and 64 invalid items
+(32 signatures over another message, 16 x-coordinates without a square root, 16
+on-curve points outside the r-order subgroup).  Signatures arrive compressed (48 B) and
+are decompressed and validated on the device; -H(m) arrives in ic_bls12_381's
+Montgomery form; the 4096 per-key line tables are resident in HBM (registered once,
+outside the timed region), as are all inputs when the timed region starts.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the test oracle (a C
+restatement of the reference's CPU path) timed on the host's cores on a bounded sample
+of the same batch; its verdicts are also compared with the GPU's on that sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "icp-gpu_amd"))
+
+# algorithmic work per verification as the reference implements it (Fp multiplications)
+FP_MULS_MILLER = 8080
+FP_MULS_FINAL_EXP = 8861
+FP_MULS_DECOMPRESS = 616
+PRODUCTS_PER_FP_MUL = 288  # 12x12 limb products + 12x12 reduction products (32-bit limbs)
+MI355X_CUS, MI355X_GHZ = 256, 2.4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def make_workload(lib, n, seed):
+    """uniform batch: n distinct keys, item i signed by key i, 64 invalid (if n >= 128)"""
+    rng = np.random.default_rng(seed)
+    cat = np.zeros(n, np.int8)
+    n_bad = 64 if n >= 128 else max(n // 16, 0)
+    if n_bad:
+        pos = rng.choice(n, size=n_bad, replace=False)
+        half, q = n_bad // 2, n_bad // 4
+        cat[pos[:half]] = 1            # wrong message
+        cat[pos[half:half + q]] = 2    # x^3+4 not a square
+        cat[pos[half + q:]] = 3        # off the subgroup
+    sk = np.zeros((n, 32), np.uint8)
+    lib.blsgpu_wl_secret_keys(ctypes.c_uint64(seed), n, sk.ctypes.data_as(ctypes.c_void_p))
+    pk = np.zeros((n, 24), np.uint64)
+    if lib.blsgpu_wl_public_keys(sk.ctypes.data_as(ctypes.c_void_p), n, pk.ctypes.data_as(ctypes.c_void_p)):
+        raise RuntimeError("public keys: " + lib.blsgpu_wl_last_error().decode())
+    key_idx = np.arange(n, dtype=np.int32)
+    sig = np.zeros((n, 48), np.uint8)
+    nhm = np.zeros((n, 12), np.uint64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    if lib.blsgpu_wl_items(ctypes.c_uint64(seed), n, p(sk), n, p(key_idx), p(cat), p(sig), p(nhm)):
+        raise RuntimeError("items: " + lib.blsgpu_wl_last_error().decode())
+    return sk, pk, key_idx, cat, sig, nhm
+
+
+def expected_codes(cat):
+    return np.select([cat == 0, cat == 1, cat == 2, cat == 3], [1, 0, -3, -4]).astype(np.int32)
+
+
+def cpu_baseline(sig, nhm, pk, key_idx, gpu_res, budget_s):
+    """Oracle (CPU restatement of the reference's path) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    g2c = O.g2_prepare(O.g2_gen())
+    # one single-thread timing to size the sample
+    def run(idx, threads):
+        keys = np.unique(key_idx[idx])
+        remap = {k: j for j, k in enumerate(keys)}
+        kc = np.ascontiguousarray(np.array([O.g2_prepare(pk[k]) for k in keys]))
+        ki = np.array([remap[k] for k in key_idx[idx]], np.int32)
+        s = np.ascontiguousarray(sig[idx])
+        h = np.ascontiguousarray(nhm[idx])
+        res = np.zeros(len(idx), np.int32)
+        t0 = time.perf_counter()
+        O.lib().orc_verify_compressed_batch(O.ptr(s), O.ptr(h), O.ptr(g2c), O.ptr(kc), O.ptr(ki), O.ptr(res),
+                                           len(idx), threads)
+        return res, time.perf_counter() - t0
+    _, t1 = run(np.arange(2), 1)
+    per = t1 / 2
+    n_sample = int(min(len(sig), max(cores, budget_s * cores / per)))
+    n_sample -= n_sample % cores if n_sample > cores else 0
+    idx = np.arange(n_sample)
+    res, t = run(idx, cores)
+    agree = bool((res == gpu_res[idx]).all())
+    return {
+        "value": n_sample / t,
+        "unit": "verifications/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "first %d items of the same 4096-item batch (decompress+checks+verify), %.1f s; "
+                  "single-thread %.2f ms/verify; verdicts agree with GPU: %s" % (n_sample, t, per * 1e3, agree),
+        "single_thread_ms": per * 1e3,
+        "verdicts_agree": agree,
+    }
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (no multi-GPU path); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    log("importing torch")
+    import torch
+    import blsgpu
+
+    lib = blsgpu.load()
+    wl = ctypes.CDLL(blsgpu.LIB_PATH)
+    wl.blsgpu_wl_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    wl.blsgpu_probe_mad_rate.argtypes = [vp, vp]
+    lib.blsgpu_verify_compressed_device.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp]
+    lib.blsgpu_phase_times.argtypes = [vp, vp]
+    if blsgpu.init() != 0:
+        raise RuntimeError("no GPU: " + blsgpu.last_error())
+
+    n = args.batch
+    log("generating workload n=%d seed=%d" % (n, args.seed))
+    t_gen = time.perf_counter()
+    sk, pk, key_idx, cat, sig, nhm = make_workload(wl, n, args.seed)
+    lib.blsgpu_reset_keys()
+    handles = np.array([blsgpu.register_key_affine(pk[i]) for i in range(n)], np.int32)
+    t_gen = time.perf_counter() - t_gen
+
+    dev = torch.device("cuda:0")
+    d_sig = torch.from_numpy(sig.reshape(-1)).to(dev)
+    d_nhm = torch.from_numpy(nhm.view(np.int64).reshape(-1)).to(dev)
+    d_key = torch.from_numpy(handles[key_idx]).to(dev)
+    d_res = torch.zeros(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def step():
+        rc = lib.blsgpu_verify_compressed_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(),
+                                                 d_res.data_ptr(), None, n, sp)
+        if rc != 0:
+            raise RuntimeError(blsgpu.last_error())
+
+    log("warmup %d steps" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.blsgpu_set_profiling(1)
+    torch.cuda.synchronize()
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phase = np.zeros(4, np.float64)
+    calls = ctypes.c_int(0)
+    lib.blsgpu_phase_times(phase.ctypes.data_as(vp), ctypes.byref(calls))
+    lib.blsgpu_set_profiling(0)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_per_step * 1e-3)
+
+    res = d_res.cpu().numpy()
+    want = expected_codes(cat)
+    ok = bool((res == want).all())
+
+    # roofline of the dominant kernel: algorithmic 32-bit limb products / duration
+    phase_avg = phase / max(calls.value, 1)
+    names = ["g1_decompress", "neg_hm_convert", "miller_loop", "final_exp"]
+    decoded = int((want >= 0).sum())
+    work = [n * FP_MULS_DECOMPRESS, 0, decoded * FP_MULS_MILLER, decoded * FP_MULS_FINAL_EXP]
+    dom = int(np.argmax(phase_avg))
+    mads = ctypes.c_double(0)
+    pms = ctypes.c_double(0)
+    wl.blsgpu_probe_mad_rate(ctypes.byref(mads), ctypes.byref(pms))
+    achieved = work[dom] * PRODUCTS_PER_FP_MUL / (phase_avg[dom] * 1e-3)
+    roofline = {
+        "bound": "valu",
+        "kernel": names[dom],
+        "achieved": achieved / 1e9,
+        "peak": mads.value / 1e9,
+        "unit": "G(32x32-bit products)/s",
+        "frac": achieved / mads.value,
+        "traffic": None,
+        "peak_theoretical": MI355X_CUS * 64 * MI355X_GHZ,
+        "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, phase_avg)},
+        "whole_job_frac": value * (FP_MULS_MILLER + FP_MULS_FINAL_EXP) * PRODUCTS_PER_FP_MUL / mads.value,
+    }
+
+    out = {
+        "metric": "BLS sig verifications/sec on 1 GPU, batched vs plain; p50/p99 latency at load",
+        "value": value,
+        "unit": "verifications/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 limbs (exact 381-bit integer)",
+        "data": "synthetic: seeded keys/messages generated on the device (seed %d)" % args.seed,
+        "config": {"workload": "plain GPU form, uniform batch of %d verifications, %d distinct keys, "
+                               "64 invalid (32 wrong-message, 16 no-sqrt, 16 off-subgroup), "
+                               "compressed signatures, inputs resident in HBM" % (n, n),
+                   "global_batch": n, "parallelism": "single GPU"},
+        "verdicts_as_expected": ok,
+        "roofline": roofline,
+        "setup_s": round(t_gen, 2),
+    }
+    if not args.no_cpu:
+        log("cpu baseline (budget %.0f s)" % args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(sig, nhm, pk, key_idx, res, args.cpu_seconds)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -82,9 +82,23 @@ struct Ctx {
   DevBuf gen_lines;  // G2 generator lines (internal), computed at init
   DevBuf keys;       // registered key tables (internal, key-major)
   int nkeys = 0;
-  std::vector<uint64_t> host_tmp;
+  // device-form calls: the stream the workspace was last used on, and timing events
+  hipStream_t last = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_free;
+  std::vector<std::vector<hipEvent_t>> ev_pending;
+  double phase_ms[4] = {0, 0, 0, 0};
+  int phase_calls = 0;
 };
 Ctx g;
+
+// Workspace buffers are shared by all calls; before they are reused (or grown) from a
+// different stream, wait for the stream that last used them.
+int claim_workspace_locked(hipStream_t s) {
+  if (g.last && g.last != s) HIPCHK(hipStreamSynchronize(g.last));
+  g.last = s;
+  return 0;
+}
 
 int init_locked() {
   if (g.state == 1) return 0;
@@ -110,6 +124,7 @@ int init_locked() {
 int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint64_t* g2c,
                          const uint64_t* pkc, int32_t* results, uint64_t* gt_out, int M) {
   CHK(init_locked());
+  CHK(claim_workspace_locked(g.s));
   const size_t m = (size_t)M;
   CHK(g.sig_ref.ensure(m * 96));
   CHK(g.nhm_ref.ensure(m * 96));
@@ -149,6 +164,7 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
 int grow_keys_locked(int add) {
   size_t need = (size_t)(g.nkeys + add) * bls::KEY_TABLE_WORDS * 4;
   if (need <= g.keys.cap) return 0;
+  CHK(claim_workspace_locked(g.s));
   size_t want = need * 2;
   void* np = nullptr;
   HIPCHK(hipMalloc(&np, want));
@@ -206,6 +222,7 @@ int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* st
     return -1;
   }
   CHK(init_locked());
+  CHK(claim_workspace_locked(g.s));
   const size_t m = (size_t)n;
   CHK(g.comp.ensure(m * 48));
   CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
@@ -311,6 +328,7 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
     return -1;
   }
   CHK(init_locked());
+  CHK(claim_workspace_locked(g.s));
   for (int i = 0; i < M; i++)
     if (key_idx[i] < 0 || key_idx[i] >= g.nkeys) {
       g_err = "key index out of range";
@@ -335,6 +353,7 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   a.nhm_ref = g.nhm_ref.as<uint64_t>();
   a.key_idx = g.keyidx.as<int32_t>();
   a.keys = g.keys.as<int32_t>();
+  a.nkeys = g.nkeys;
   a.gen_lines = g.gen_lines.as<int32_t>();
   a.sig_pts = g.sig_pts.as<int32_t>();
   a.nhm_pts = g.nhm_pts.as<int32_t>();
@@ -346,6 +365,82 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   HIPCHK(hipMemcpyAsync(results, g.res.p, m * 4, hipMemcpyDeviceToHost, g.s));
   if (gt_out) HIPCHK(hipMemcpyAsync(gt_out, g.gt.p, m * 72 * 8, hipMemcpyDeviceToHost, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
+  return 0;
+}
+
+int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx,
+                                    int32_t* d_results, uint64_t* d_gt, int M, void* stream) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (M <= 0 || !d_sigs48 || !d_neg_hms || !d_key_idx || !d_results) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  CHK(init_locked());
+  hipStream_t s = stream ? (hipStream_t)stream : g.s;
+  CHK(claim_workspace_locked(s));
+  const size_t m = (size_t)M;
+  CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.nhm_pts.ensure(m * bls::POINT_WORDS * 4));
+  CHK(g.status.ensure(m * 4));
+  CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
+  bls::EngineArgs a{};
+  a.M = M;
+  a.comp = d_sigs48;
+  a.nhm_ref = d_neg_hms;
+  a.key_idx = d_key_idx;
+  a.keys = g.keys.as<int32_t>();
+  a.nkeys = g.nkeys;
+  a.gen_lines = g.gen_lines.as<int32_t>();
+  a.sig_pts = g.sig_pts.as<int32_t>();
+  a.nhm_pts = g.nhm_pts.as<int32_t>();
+  a.status = g.status.as<int32_t>();
+  a.fbuf = g.fbuf.as<int32_t>();
+  a.results = d_results;
+  a.gt = d_gt;
+  std::vector<hipEvent_t> ev;
+  if (g.profiling) {
+    for (int k = 0; k < 5; k++) {
+      hipEvent_t e;
+      if (!g.ev_free.empty()) {
+        e = g.ev_free.back();
+        g.ev_free.pop_back();
+      } else {
+        HIPCHK(hipEventCreate(&e));
+      }
+      ev.push_back(e);
+    }
+    a.ev = ev.data();
+  }
+  CHK(bls::engine_compressed(a, s));
+  if (g.profiling) g.ev_pending.push_back(ev);
+  return 0;
+}
+
+int blsgpu_set_profiling(int on) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g.profiling = on != 0;
+  for (int k = 0; k < 4; k++) g.phase_ms[k] = 0;
+  g.phase_calls = 0;
+  return 0;
+}
+
+int blsgpu_phase_times(double ms[4], int* calls) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  for (auto& ev : g.ev_pending) {
+    HIPCHK(hipEventSynchronize(ev[4]));
+    for (int k = 0; k < 4; k++) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      g.phase_ms[k] += t;
+    }
+    g.phase_calls++;
+    for (hipEvent_t e : ev) g.ev_free.push_back(e);
+  }
+  g.ev_pending.clear();
+  for (int k = 0; k < 4; k++) ms[k] = g.phase_ms[k];
+  if (calls) *calls = g.phase_calls;
   return 0;
 }
 

@@ -113,6 +113,7 @@ enum DecodeStatus : int32_t {
   DEC_X_NOT_CANONICAL = -2,
   DEC_NOT_ON_CURVE = -3,
   DEC_NOT_IN_SUBGROUP = -4,
+  DEC_BAD_KEY = -5,  // set by the Miller kernel: key index outside the registry
 };
 // Decode 48 big-endian bytes given as 12 big-endian-packed 32-bit words (w[0] holds bytes
 // 0..3).  Semantics of ic_bls12_381 G1Affine::from_compressed: the reference GPU kernel

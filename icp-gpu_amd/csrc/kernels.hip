@@ -164,8 +164,8 @@ __global__ void k_points_to_ref(const i32* __restrict__ pts, const i32* __restri
 // selected by key_idx[i].  sig_status (nullable): DEC_IDENTITY drops pair 1, any other
 // non-OK status skips the item (its verdict is set by the final kernel).
 __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nhm, const i32* __restrict__ g2lines,
-                         const i32* __restrict__ pklines, const int32_t* __restrict__ key_idx,
-                         const i32* __restrict__ sig_status, i32* __restrict__ fout, int M) {
+                         const i32* __restrict__ pklines, const int32_t* __restrict__ key_idx, int nkeys,
+                         i32* __restrict__ sig_status, i32* __restrict__ fout, int M) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   bool skip1 = false;
@@ -173,6 +173,10 @@ __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nh
     int32_t st = sig_status[i];
     if (st != DEC_OK && st != DEC_IDENTITY) return;
     skip1 = st == DEC_IDENTITY;
+  }
+  if (key_idx && (key_idx[i] < 0 || key_idx[i] >= nkeys)) {
+    if (sig_status) sig_status[i] = DEC_BAD_KEY;
+    return;
   }
   G1Aff p1 = {ld_fp_soa(sig, 0, M, i), ld_fp_soa(sig, NL, M, i)};
   G1Aff p2 = {ld_fp_soa(nhm, 0, M, i), ld_fp_soa(nhm, NL, M, i)};
@@ -257,10 +261,10 @@ int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* ou
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines, const int32_t* pklines,
-                  const int32_t* key_idx, const int32_t* sig_status, int32_t* fout, int M, int block,
+                  const int32_t* key_idx, int nkeys, int32_t* sig_status, int32_t* fout, int M, int block,
                   hipStream_t s) {
   hipLaunchKernelGGL(k_miller, dim3(nblk(M, block)), dim3(block), 0, s, sig, nhm, g2lines, pklines, key_idx,
-                     sig_status, fout, M);
+                     nkeys, sig_status, fout, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,

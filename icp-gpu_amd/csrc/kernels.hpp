@@ -18,7 +18,7 @@ int launch_g1_decompress(const uint8_t* in, int32_t* pts, int32_t* status, int M
                          hipStream_t s);
 int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* out, int M, hipStream_t s);
 int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines, const int32_t* pklines,
-                  const int32_t* key_idx, const int32_t* sig_status, int32_t* fout, int M, int block,
+                  const int32_t* key_idx, int nkeys, int32_t* sig_status, int32_t* fout, int M, int block,
                   hipStream_t s);
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
                      int block, hipStream_t s);

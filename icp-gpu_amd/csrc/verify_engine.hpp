@@ -16,6 +16,7 @@ struct EngineArgs {
   const uint64_t* pk_ref = nullptr;   // legacy: M x 2448 u64 per-item lines
   const int32_t* key_idx = nullptr;   // fused: M key handles
   const int32_t* keys = nullptr;      // fused: resident key tables (internal)
+  int nkeys = 0;                      // fused: number of resident keys
   const int32_t* gen_lines = nullptr; // fused: generator table (internal)
   // workspace
   int32_t* sig_pts = nullptr;   // M x 28
@@ -29,6 +30,8 @@ struct EngineArgs {
   uint64_t* gt = nullptr;      // M x 72 (nullable)
   // tuning
   int block = 64;
+  // optional timing: events recorded before each phase and at the end (5 events)
+  hipEvent_t* ev = nullptr;
 };
 
 int engine_legacy(const EngineArgs& a, hipStream_t s);

@@ -96,6 +96,21 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
 int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* status, int n,
                                 int check_subgroup);
 
+/* Device-resident form of blsgpu_verify_compressed: every array is device memory and
+ * the work is enqueued on `stream` (a hipStream_t; NULL = the library's stream) without
+ * waiting.  Items whose key index is out of range get BLSGPU_ERR_BAD_KEY.  Workspace
+ * comes from the library's pool; calls are serialized by the library's lock. */
+#define BLSGPU_ERR_BAD_KEY (-5)
+int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms,
+                                    const int32_t* d_key_idx, int32_t* d_results, uint64_t* d_gt,
+                                    int M, void* stream);
+
+/* Per-kernel timing with HIP events recorded between the kernels of every device-form
+ * call while enabled.  phases: 0 decompress, 1 -H(m) conversion, 2 Miller loop,
+ * 3 final exponentiation.  ms[k] = summed milliseconds; *calls = calls accumulated. */
+int blsgpu_set_profiling(int on);
+int blsgpu_phase_times(double ms[4], int* calls);
+
 /* Last error message of the calling thread ("" if none). */
 const char* blsgpu_last_error(void);
 

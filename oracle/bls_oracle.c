@@ -801,6 +801,53 @@ int orc_verify_batch_keyed(const u64* sigs, const u64* nhms, const u64* g2c, con
     return run_batch(b, threads);
 }
 
+/* fused CPU path: decompress (with all checks) + verify against keyed lines.
+ * results as the GPU's fused path: 1 valid, 0 invalid, negative decode code. */
+typedef struct {
+    const uint8_t* sig48; const u64 *nhms, *g2c, *key_coeffs; const int32_t* key_idx;
+    int32_t* res; int m, stride, tid;
+} cjob_t;
+static void* cverify_worker(void* arg) {
+    cjob_t* j = (cjob_t*)arg;
+    for (int i = j->tid; i < j->m; i += j->stride) {
+        u64 pt[12];
+        int rc = orc_g1_decompress(j->sig48 + (size_t)i * 48, pt);
+        if (rc == ORC_DEC_IDENTITY) {
+            /* identity signature: e(O, G2) = 1, so only the second pair remains */
+            fp12 ml, r;
+            LD(g1a, n, j->nhms + (size_t)i * 12);
+            const fp2(*pk)[3] = (const fp2(*)[3])(j->key_coeffs + (size_t)j->key_idx[i] * 2448);
+            u64 tmp[72];
+            orc_miller1(j->nhms + (size_t)i * 12, (const u64*)pk, tmp);
+            (void)n;
+            memcpy(&ml, tmp, sizeof ml);
+            final_exp(&r, &ml);
+            j->res[i] = fp12_is_one(&r);
+            continue;
+        }
+        if (rc != ORC_DEC_OK) { j->res[i] = rc; continue; }
+        j->res[i] = orc_verify(pt, j->nhms + (size_t)i * 12, j->g2c,
+                               j->key_coeffs + (size_t)j->key_idx[i] * 2448, NULL);
+    }
+    return NULL;
+}
+int orc_verify_compressed_batch(const uint8_t* sig48, const u64* nhms, const u64* g2c, const u64* key_coeffs,
+                                const int32_t* key_idx, int32_t* results, int m, int threads) {
+    if (threads <= 0) threads = orc_num_threads();
+    if (threads > 256) threads = 256;
+    if (threads > m) threads = m > 0 ? m : 1;
+    pthread_t th[256];
+    cjob_t jobs[256];
+    for (int t = 0; t < threads; t++) {
+        cjob_t b = {sig48, nhms, g2c, key_coeffs, key_idx, results, m, threads, t};
+        jobs[t] = b;
+        if (t > 0) pthread_create(&th[t], NULL, cverify_worker, &jobs[t]);
+    }
+    cverify_worker(&jobs[0]);
+    for (int t = 1; t < threads; t++) pthread_join(th[t], NULL);
+    return 0;
+}
+
 /* ---- G1 encoding (ZCash format: bit7 compressed, bit6 infinity, bit5 sort) ---- */
 static void fp_to_be(uint8_t out[48], const fp* a) {
     fp n; fp_from_mont(&n, a);

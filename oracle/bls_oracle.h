@@ -80,6 +80,12 @@ int  orc_verify_batch_keyed(const uint64_t* sigs, const uint64_t* neg_hms, const
                             const uint64_t* key_coeffs, const int32_t* key_idx,
                             int32_t* results, uint64_t* gts, int m, int threads);
 
+/* fused CPU path: decompress each 48-byte signature with all checks, then verify;
+ * results: 1 valid, 0 invalid, negative ORC_DEC_* code for a rejected encoding. */
+int  orc_verify_compressed_batch(const uint8_t* sig48, const uint64_t* neg_hms, const uint64_t* g2c,
+                                 const uint64_t* key_coeffs, const int32_t* key_idx,
+                                 int32_t* results, int m, int threads);
+
 /* ---- groups (affine Montgomery; identity encoded by *inf / return code) ---- */
 #define ORC_DEC_OK 0
 #define ORC_DEC_IDENTITY 1
