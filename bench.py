@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine", default="1:16:16", help="mode:lanes_miller:lanes_final_exp")
+    ap.add_argument("--sweep", action="store_true", help="time every engine configuration first")
     return ap.parse_args()
 
 
@@ -158,6 +160,23 @@ def main():
         if rc != 0:
             raise RuntimeError(blsgpu.last_error())
 
+    if args.sweep:
+        timings = {}
+        for cfg in [(1, 8, 16), (1, 16, 16), (1, 32, 16), (1, 16, 32), (1, 32, 32)]:
+            blsgpu.set_engine(*cfg)
+            step()
+            torch.cuda.synchronize()
+            t_a = time.perf_counter()
+            for _ in range(3):
+                step()
+            torch.cuda.synchronize()
+            timings["%d:%d:%d" % cfg] = (time.perf_counter() - t_a) * 1e3 / 3
+            log("sweep %s: %.3f ms/step" % ("%d:%d:%d" % cfg, timings["%d:%d:%d" % cfg]))
+        best = min(timings, key=timings.get)
+        args.engine = best
+    mode, gml, gfe = (int(x) for x in args.engine.split(":"))
+    blsgpu.set_engine(mode, gml, gfe)
+    log("engine %s" % args.engine)
     log("warmup %d steps" % args.warmup)
     for _ in range(args.warmup):
         step()
@@ -224,6 +243,7 @@ def main():
         "verdicts_as_expected": ok,
         "roofline": roofline,
         "setup_s": round(t_gen, 2),
+        "engine": {"mode": mode, "lanes_miller": gml, "lanes_final_exp": gfe},
     }
     if not args.no_cpu:
         log("cpu baseline (budget %.0f s)" % args.cpu_seconds)

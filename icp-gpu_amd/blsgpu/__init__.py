@@ -35,6 +35,11 @@ EXTENSION_SYMBOLS = (
     "blsgpu_verify_compressed",
     "blsgpu_g1_decompress_status",
     "blsgpu_last_error",
+    "blsgpu_set_engine",
+    "blsgpu_get_engine",
+    "blsgpu_verify_compressed_device",
+    "blsgpu_set_profiling",
+    "blsgpu_phase_times",
 )
 
 VALID, INVALID = 1, 0
@@ -77,6 +82,11 @@ def load(path=None):
         "blsgpu_verify_compressed": ([vp, vp, vp, vp, vp, i32], i32),
         "blsgpu_g1_decompress_status": ([vp, vp, vp, i32, i32], i32),
         "blsgpu_last_error": ([], ctypes.c_char_p),
+        "blsgpu_set_engine": ([i32, i32, i32], i32),
+        "blsgpu_get_engine": ([vp, vp, vp], i32),
+        "blsgpu_verify_compressed_device": ([vp, vp, vp, vp, vp, i32, vp], i32),
+        "blsgpu_set_profiling": ([i32], i32),
+        "blsgpu_phase_times": ([vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -155,3 +165,13 @@ def verify_compressed(sigs48, neg_hms, key_idx, want_gt=False):
     _check(load().blsgpu_verify_compressed(_p(sigs48), _p(neg_hms), _p(key_idx), _p(res), _p(gt), m),
            "blsgpu_verify_compressed")
     return (res, gt) if want_gt else res
+
+
+def set_engine(mode, g_miller=16, g_final_exp=16):
+    _check(load().blsgpu_set_engine(mode, g_miller, g_final_exp), "blsgpu_set_engine")
+
+
+def get_engine():
+    m, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    load().blsgpu_get_engine(ctypes.byref(m), ctypes.byref(a), ctypes.byref(b))
+    return m.value, a.value, b.value

@@ -89,6 +89,8 @@ struct Ctx {
   std::vector<std::vector<hipEvent_t>> ev_pending;
   double phase_ms[4] = {0, 0, 0, 0};
   int phase_calls = 0;
+  // engine configuration (blsgpu_set_engine)
+  int mode = 1, g_ml = 16, g_fe = 16;
 };
 Ctx g;
 
@@ -143,6 +145,9 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   HIPCHK(hipMemcpyAsync(g.pk_ref.p, pkc, m * 2448 * 8, hipMemcpyHostToDevice, g.s));
   bls::EngineArgs a{};
   a.M = M;
+  a.mode = g.mode;
+  a.g_ml = g.g_ml;
+  a.g_fe = g.g_fe;
   a.sig_ref = g.sig_ref.as<uint64_t>();
   a.nhm_ref = g.nhm_ref.as<uint64_t>();
   a.g2_ref = g.g2_ref.as<uint64_t>();
@@ -349,6 +354,9 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   HIPCHK(hipMemcpyAsync(g.keyidx.p, key_idx, m * 4, hipMemcpyHostToDevice, g.s));
   bls::EngineArgs a{};
   a.M = M;
+  a.mode = g.mode;
+  a.g_ml = g.g_ml;
+  a.g_fe = g.g_fe;
   a.comp = g.comp.as<uint8_t>();
   a.nhm_ref = g.nhm_ref.as<uint64_t>();
   a.key_idx = g.keyidx.as<int32_t>();
@@ -386,6 +394,9 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   bls::EngineArgs a{};
   a.M = M;
+  a.mode = g.mode;
+  a.g_ml = g.g_ml;
+  a.g_fe = g.g_fe;
   a.comp = d_sigs48;
   a.nhm_ref = d_neg_hms;
   a.key_idx = d_key_idx;
@@ -414,6 +425,31 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
   }
   CHK(bls::engine_compressed(a, s));
   if (g.profiling) g.ev_pending.push_back(ev);
+  return 0;
+}
+
+int blsgpu_set_engine(int mode, int g_miller, int g_final_exp) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  const bool ok_ml = g_miller == 8 || g_miller == 16 || g_miller == 32;
+  const bool ok_fe = g_final_exp == 16 || g_final_exp == 32;
+  if ((mode != 0 && mode != 1) || (mode == 1 && (!ok_ml || !ok_fe))) {
+    g_err = "unsupported engine configuration";
+    return -1;
+  }
+  g.mode = mode;
+  if (mode == 1) {
+    g.g_ml = g_miller;
+    g.g_fe = g_final_exp;
+  }
+  return 0;
+}
+
+int blsgpu_get_engine(int* mode, int* g_miller, int* g_final_exp) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (mode) *mode = g.mode;
+  if (g_miller) *g_miller = g.g_ml;
+  if (g_final_exp) *g_final_exp = g.g_fe;
   return 0;
 }
 

@@ -23,4 +23,12 @@ int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
                      int block, hipStream_t s);
 
+// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 16, 32;
+// final exponentiation: 16, 32).  Return -1 for an unsupported G or a launch error.
+int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
+                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                      hipStream_t s);
+int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, int32_t* result, uint64_t* gt, int M,
+                         hipStream_t s);
+
 }  // namespace bls

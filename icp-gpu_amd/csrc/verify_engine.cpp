@@ -5,14 +5,25 @@
 
 namespace bls {
 
+static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, const int32_t* key_idx,
+                  const int32_t* pk_soa, int32_t* status, hipStream_t s) {
+  if (a.mode == 0)
+    return launch_miller(a.sig_pts, a.nhm_pts, gen, key_idx ? keys : pk_soa, key_idx, a.nkeys, status, a.fbuf, a.M,
+                         a.block, s);
+  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_soa, status, a.fbuf, a.M, s);
+}
+static int final_exp(const EngineArgs& a, const int32_t* status, hipStream_t s) {
+  if (a.mode == 0) return launch_final_exp(a.fbuf, status, a.results, a.gt, a.M, a.block, s);
+  return launch_final_exp_grp(a.g_fe, a.fbuf, status, a.results, a.gt, a.M, s);
+}
+
 int engine_legacy(const EngineArgs& a, hipStream_t s) {
   if (launch_points_from_ref(a.sig_ref, a.sig_pts, a.M, s)) return -1;
   if (launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
   if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, 0, s)) return -1;
   if (launch_lines_from_ref(a.pk_ref, a.pk_lines, a.M, 1, s)) return -1;
-  if (launch_miller(a.sig_pts, a.nhm_pts, a.g2_lines, a.pk_lines, nullptr, 0, nullptr, a.fbuf, a.M, a.block, s))
-    return -1;
-  return launch_final_exp(a.fbuf, nullptr, a.results, a.gt, a.M, a.block, s);
+  if (miller(a, a.g2_lines, nullptr, nullptr, a.pk_lines, nullptr, s)) return -1;
+  return final_exp(a, nullptr, s);
 }
 
 static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
@@ -22,11 +33,8 @@ static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
 int engine_compressed(const EngineArgs& a, hipStream_t s) {
   if (mark(a, 0, s) || launch_g1_decompress(a.comp, a.sig_pts, a.status, a.M, 1, s)) return -1;
   if (mark(a, 1, s) || launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
-  if (mark(a, 2, s) ||
-      launch_miller(a.sig_pts, a.nhm_pts, a.gen_lines, a.keys, a.key_idx, a.nkeys, a.status, a.fbuf, a.M,
-                    a.block, s))
-    return -1;
-  if (mark(a, 3, s) || launch_final_exp(a.fbuf, a.status, a.results, a.gt, a.M, a.block, s)) return -1;
+  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s)) return -1;
+  if (mark(a, 3, s) || final_exp(a, a.status, s)) return -1;
   return mark(a, 4, s);
 }
 

@@ -28,8 +28,11 @@ struct EngineArgs {
   // outputs
   int32_t* results = nullptr;  // M
   uint64_t* gt = nullptr;      // M x 72 (nullable)
-  // tuning
+  // tuning: mode 0 = one thread per item, 1 = G lanes per item (g_ml Miller, g_fe final exp)
   int block = 64;
+  int mode = 1;
+  int g_ml = 16;
+  int g_fe = 16;
   // optional timing: events recorded before each phase and at the end (5 events)
   hipEvent_t* ev = nullptr;
 };
