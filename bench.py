@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", default="1:16:16", help="mode:lanes_miller:lanes_final_exp")
     ap.add_argument("--sweep", action="store_true", help="time every engine configuration first")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
     return ap.parse_args()
 
 
@@ -162,7 +164,7 @@ def main():
 
     if args.sweep:
         timings = {}
-        for cfg in [(1, 8, 16), (1, 16, 16), (1, 32, 16), (1, 16, 32), (1, 32, 32)]:
+        for cfg in [(1, 12, 16), (1, 13, 16), (1, 16, 16), (1, 13, 12), (1, 13, 18), (1, 16, 18)]:
             blsgpu.set_engine(*cfg)
             step()
             torch.cuda.synchronize()
@@ -196,6 +198,12 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3)
 
+    if args.copy_probe:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     res = d_res.cpu().numpy()
     want = expected_codes(cat)
     ok = bool((res == want).all())

@@ -77,7 +77,9 @@ struct Ctx {
   hipStream_t s = nullptr;
   // per-call staging
   DevBuf sig_ref, nhm_ref, g2_ref, pk_ref, comp, keyidx;
-  DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, fbuf, res, gt;
+  DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, sub_status, fbuf, res, gt;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // resident
   DevBuf gen_lines;  // G2 generator lines (internal), computed at init
   DevBuf keys;       // registered key tables (internal, key-major)
@@ -113,6 +115,9 @@ int init_locked() {
   }
   HIPCHK(hipSetDevice(g.device));
   HIPCHK(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
   // generator lines, once
   CHK(g.g2_ref.ensure(24 * 8));
   CHK(g.gen_lines.ensure(bls::KEY_TABLE_WORDS * 4));
@@ -232,10 +237,18 @@ int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* st
   CHK(g.comp.ensure(m * 48));
   CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
   CHK(g.status.ensure(m * 4));
+  CHK(g.sub_status.ensure(m * 4));
   CHK(g.sig_ref.ensure(m * 96));
   HIPCHK(hipMemcpyAsync(g.comp.p, in48, m * 48, hipMemcpyHostToDevice, g.s));
-  CHK(bls::launch_g1_decompress(g.comp.as<uint8_t>(), g.sig_pts.as<int32_t>(), g.status.as<int32_t>(), n,
-                                check_subgroup, g.s));
+  {
+    bls::EngineArgs a{};
+    a.M = n;
+    a.comp = g.comp.as<uint8_t>();
+    a.sig_pts = g.sig_pts.as<int32_t>();
+    a.status = g.status.as<int32_t>();
+    a.sub_status = g.sub_status.as<int32_t>();
+    CHK(bls::engine_decompress(a, check_subgroup, g.s));
+  }
   CHK(bls::launch_points_to_ref(g.sig_pts.as<int32_t>(), g.status.as<int32_t>(), g.sig_ref.as<uint64_t>(), n, g.s));
   if (out96) HIPCHK(hipMemcpyAsync(out96, g.sig_ref.p, m * 96, hipMemcpyDeviceToHost, g.s));
   if (status) HIPCHK(hipMemcpyAsync(status, g.status.p, m * 4, hipMemcpyDeviceToHost, g.s));
@@ -346,6 +359,7 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
   CHK(g.nhm_pts.ensure(m * bls::POINT_WORDS * 4));
   CHK(g.status.ensure(m * 4));
+  CHK(g.sub_status.ensure(m * 4));
   CHK(g.res.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
@@ -366,6 +380,10 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   a.sig_pts = g.sig_pts.as<int32_t>();
   a.nhm_pts = g.nhm_pts.as<int32_t>();
   a.status = g.status.as<int32_t>();
+  a.sub_status = g.sub_status.as<int32_t>();
+  a.s2 = g.s2;
+  a.fork = g.ev_fork;
+  a.join = g.ev_join;
   a.fbuf = g.fbuf.as<int32_t>();
   a.results = g.res.as<int32_t>();
   a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
@@ -391,6 +409,7 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
   CHK(g.sig_pts.ensure(m * bls::POINT_WORDS * 4));
   CHK(g.nhm_pts.ensure(m * bls::POINT_WORDS * 4));
   CHK(g.status.ensure(m * 4));
+  CHK(g.sub_status.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   bls::EngineArgs a{};
   a.M = M;
@@ -406,6 +425,10 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
   a.sig_pts = g.sig_pts.as<int32_t>();
   a.nhm_pts = g.nhm_pts.as<int32_t>();
   a.status = g.status.as<int32_t>();
+  a.sub_status = g.sub_status.as<int32_t>();
+  a.s2 = g.s2;
+  a.fork = g.ev_fork;
+  a.join = g.ev_join;
   a.fbuf = g.fbuf.as<int32_t>();
   a.results = d_results;
   a.gt = d_gt;
@@ -431,8 +454,8 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
 int blsgpu_set_engine(int mode, int g_miller, int g_final_exp) {
   std::lock_guard<std::mutex> lk(g.mu);
   g_err.clear();
-  const bool ok_ml = g_miller == 8 || g_miller == 16 || g_miller == 32;
-  const bool ok_fe = g_final_exp == 16 || g_final_exp == 32;
+  const bool ok_ml = g_miller == 8 || g_miller == 12 || g_miller == 13 || g_miller == 16 || g_miller == 32;
+  const bool ok_fe = g_final_exp == 12 || g_final_exp == 16 || g_final_exp == 18 || g_final_exp == 32;
   if ((mode != 0 && mode != 1) || (mode == 1 && (!ok_ml || !ok_fe))) {
     g_err = "unsupported engine configuration";
     return -1;

@@ -123,11 +123,13 @@ __device__ __forceinline__ Fp carry(const Fp& a) {
 }
 
 // ---- Montgomery product  a*b*2^-392 (mod p) ----
-// Product scanning over 27 columns with the reduction interleaved (the m_k*p
-// terms of column k are added in the same pass).  Column k holds <= 14 products
-// |a_i b_j| < 2^58 (N2 operands) plus <= 14 products m_i p_j < 2^56 plus a carry
-// < 2^35, so |column| < 2^62.2: no overflow of the signed 64-bit accumulators.
-// Two accumulators per column (even/odd i) halve the dependency chain.
+// Product scanning over 27 columns with the reduction interleaved (the m_k*p terms of
+// column k are added in the same pass).  Column k holds <= 14 products |a_i b_j| < 2^58
+// (N2 operands) plus <= 14 products m_i p_j < 2^56 plus a carry < 2^35, so
+// |column| < 2^62.2: no overflow of the signed 64-bit accumulators.
+// Latency: the product sums of a column start from zero in four independent chains;
+// only the incoming carry and the term of the just-computed m_{k-1} are added after
+// them, so the cross-column dependency is a handful of instructions per column.
 // Output: limbs 0..12 in [0, 2^28); value = (ab + Mp)/2^392 with 0 <= M < 2^392, so
 // |value| < |ab|/2^392 + p < 2^380.8 for N2 inputs: N1.
 __device__ BLS_MUL_INLINE Fp mul(const Fp& a, const Fp& b) {
@@ -136,36 +138,37 @@ __device__ BLS_MUL_INLINE Fp mul(const Fp& a, const Fp& b) {
   i64 acc = 0;
 #pragma unroll
   for (int k = 0; k < NL; k++) {
-    i64 e = acc, o = 0;
+    i64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
     for (int i = 0; i <= k; i++) {
-      if (i & 1) o += (i64)a.l[i] * b.l[k - i];
-      else e += (i64)a.l[i] * b.l[k - i];
+      if (i & 1) s1 += (i64)a.l[i] * b.l[k - i];
+      else s0 += (i64)a.l[i] * b.l[k - i];
     }
 #pragma unroll
-    for (int i = 0; i < k; i++) {
-      if (i & 1) o += (i64)(i32)m[i] * P_L[k - i];
-      else e += (i64)(i32)m[i] * P_L[k - i];
+    for (int i = 0; i + 1 < k; i++) {
+      if (i & 1) s3 += (i64)(i32)m[i] * P_L[k - i];
+      else s2 += (i64)(i32)m[i] * P_L[k - i];
     }
-    i64 t = e + o;
+    i64 t = ((s0 + s1) + (s2 + s3)) + acc;
+    if (k >= 1) t += (i64)(i32)m[k - 1] * P_L[1];
     m[k] = ((u32)t * NPRIME) & (u32)MASK28;
     t += (i64)(i32)m[k] * P_L[0];
     acc = t >> 28;  // exact: t == 0 mod 2^28
   }
 #pragma unroll
   for (int k = NL; k < 2 * NL - 1; k++) {
-    i64 e = acc, o = 0;
+    i64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
     for (int i = k - (NL - 1); i < NL; i++) {
-      if (i & 1) o += (i64)a.l[i] * b.l[k - i];
-      else e += (i64)a.l[i] * b.l[k - i];
+      if (i & 1) s1 += (i64)a.l[i] * b.l[k - i];
+      else s0 += (i64)a.l[i] * b.l[k - i];
     }
 #pragma unroll
     for (int i = k - (NL - 1); i < NL; i++) {
-      if (i & 1) o += (i64)(i32)m[i] * P_L[k - i];
-      else e += (i64)(i32)m[i] * P_L[k - i];
+      if (i & 1) s3 += (i64)(i32)m[i] * P_L[k - i];
+      else s2 += (i64)(i32)m[i] * P_L[k - i];
     }
-    i64 t = e + o;
+    const i64 t = ((s0 + s1) + (s2 + s3)) + acc;
     r.l[k - NL] = (i32)t & MASK28;
     acc = t >> 28;
   }
@@ -175,7 +178,7 @@ __device__ BLS_MUL_INLINE Fp mul(const Fp& a, const Fp& b) {
 
 // Squaring: cross products a_i a_j (i != j) once, with one operand doubled.
 // |2 a_i| < 2^30, so a column holds <= 7 terms < 2^59 plus a square < 2^58 plus the
-// reduction terms: |column| < 2^62.3.
+// reduction terms: |column| < 2^62.3.  Same latency structure as mul().
 __device__ BLS_MUL_INLINE Fp sqr(const Fp& a) {
   i32 d[NL];
 #pragma unroll
@@ -185,31 +188,32 @@ __device__ BLS_MUL_INLINE Fp sqr(const Fp& a) {
   i64 acc = 0;
 #pragma unroll
   for (int k = 0; k < 2 * NL - 1; k++) {
-    i64 e = acc, o = 0;
+    i64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     const int lo = k < NL ? 0 : k - (NL - 1);
 #pragma unroll
     for (int i = lo; 2 * i < k; i++) {
-      if (i & 1) o += (i64)d[i] * a.l[k - i];
-      else e += (i64)d[i] * a.l[k - i];
+      if (i & 1) s1 += (i64)d[i] * a.l[k - i];
+      else s0 += (i64)d[i] * a.l[k - i];
     }
-    if ((k & 1) == 0) o += (i64)a.l[k / 2] * a.l[k / 2];
+    if ((k & 1) == 0) s1 += (i64)a.l[k / 2] * a.l[k / 2];
     if (k < NL) {
 #pragma unroll
-      for (int i = 0; i < k; i++) {
-        if (i & 1) o += (i64)(i32)m[i] * P_L[k - i];
-        else e += (i64)(i32)m[i] * P_L[k - i];
+      for (int i = 0; i + 1 < k; i++) {
+        if (i & 1) s3 += (i64)(i32)m[i] * P_L[k - i];
+        else s2 += (i64)(i32)m[i] * P_L[k - i];
       }
-      i64 t = e + o;
+      i64 t = ((s0 + s1) + (s2 + s3)) + acc;
+      if (k >= 1) t += (i64)(i32)m[k - 1] * P_L[1];
       m[k] = ((u32)t * NPRIME) & (u32)MASK28;
       t += (i64)(i32)m[k] * P_L[0];
       acc = t >> 28;
     } else {
 #pragma unroll
       for (int i = k - (NL - 1); i < NL; i++) {
-        if (i & 1) o += (i64)(i32)m[i] * P_L[k - i];
-        else e += (i64)(i32)m[i] * P_L[k - i];
+        if (i & 1) s3 += (i64)(i32)m[i] * P_L[k - i];
+        else s2 += (i64)(i32)m[i] * P_L[k - i];
       }
-      i64 t = e + o;
+      const i64 t = ((s0 + s1) + (s2 + s3)) + acc;
       r.l[k - NL] = (i32)t & MASK28;
       acc = t >> 28;
     }
@@ -328,6 +332,116 @@ __device__ __forceinline__ Fp pow_fixed(const Fp& a, const u32 (&e)[12]) {
 __device__ __forceinline__ Fp inv(const Fp& a) { return pow_fixed(a, INV_EXP); }
 // a^((p+1)/4); ok = (result^2 == a) (ic-integration/gpu_crypto/cuda/decompress.cu:16-40)
 __device__ __forceinline__ Fp sqrt_cand(const Fp& a) { return pow_fixed(a, SQRT_EXP); }
+
+// ---- inverse by the binary extended Euclidean algorithm ----
+// Variable time (every input here is public).  Works on the canonical 384-bit value
+// c = a R' mod p: u = c, v = p, invariants x1 c = u and x2 c = v (mod p); halving u or v
+// halves the matching x (adding p first when odd); the larger of u, v absorbs the
+// difference.  At u == 1 (or v == 1), x1 (or x2) is c^-1; the Montgomery form of a^-1
+// is c^-1 R'^2 = mont(c^-1, R'^3).  The reference inverts by Fermat exponentiation
+// (gpu-crypto/cuda/field_sppark.cuh:40-50); the result is the same field element.
+__device__ __forceinline__ bool w12_is_one(const u32 x[12]) {
+  u32 acc = x[0] ^ 1u;
+#pragma unroll
+  for (int j = 1; j < 12; j++) acc |= x[j];
+  return acc == 0;
+}
+__device__ __forceinline__ void w12_shr1(u32 x[12], u32 top) {
+#pragma unroll
+  for (int j = 0; j < 11; j++) x[j] = (x[j] >> 1) | (x[j + 1] << 31);
+  x[11] = (x[11] >> 1) | (top << 31);
+}
+// x = x / 2 mod p  (x < p)
+__device__ __forceinline__ void w12_half_mod(u32 x[12]) {
+  if (x[0] & 1u) {
+    u64 c = 0;
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+      c += (u64)x[j] + P32[j];
+      x[j] = (u32)c;
+      c >>= 32;
+    }
+    w12_shr1(x, (u32)c);
+  } else {
+    w12_shr1(x, 0u);
+  }
+}
+__device__ __forceinline__ bool w12_geq(const u32 a[12], const u32 b[12]) {
+  bool ge = true, decided = false;
+#pragma unroll
+  for (int j = 11; j >= 0; j--) {
+    if (!decided && a[j] != b[j]) {
+      ge = a[j] > b[j];
+      decided = true;
+    }
+  }
+  return ge;
+}
+// a -= b (a >= b)
+__device__ __forceinline__ void w12_sub(u32 a[12], const u32 b[12]) {
+  i64 c = 0;
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    c += (i64)a[j] - b[j];
+    a[j] = (u32)c;
+    c >>= 32;
+  }
+}
+// a = a - b mod p  (a, b < p)
+__device__ __forceinline__ void w12_sub_mod(u32 a[12], const u32 b[12]) {
+  i64 c = 0;
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    c += (i64)a[j] - b[j];
+    a[j] = (u32)c;
+    c >>= 32;
+  }
+  if (c < 0) {
+    u64 d = 0;
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+      d += (u64)a[j] + P32[j];
+      a[j] = (u32)d;
+      d >>= 32;
+    }
+  }
+}
+// 0 maps to 0
+__device__ __noinline__ Fp inv_bgcd(const Fp& a) {
+  u32 u[12], v[12], x1[12], x2[12];
+  pack384(canon(a), u);
+  u32 nz = 0;
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    nz |= u[j];
+    v[j] = P32[j];
+    x1[j] = j == 0;
+    x2[j] = 0;
+  }
+  if (nz == 0) return fp_zero();
+  while (!w12_is_one(u) && !w12_is_one(v)) {
+    while ((u[0] & 1u) == 0) {
+      w12_shr1(u, 0u);
+      w12_half_mod(x1);
+    }
+    while ((v[0] & 1u) == 0) {
+      w12_shr1(v, 0u);
+      w12_half_mod(x2);
+    }
+    if (w12_geq(u, v)) {
+      w12_sub(u, v);
+      w12_sub_mod(x1, x2);
+    } else {
+      w12_sub(v, u);
+      w12_sub_mod(x2, x1);
+    }
+  }
+  const u32* r = w12_is_one(u) ? x1 : x2;
+  u32 rr[12];
+#pragma unroll
+  for (int j = 0; j < 12; j++) rr[j] = r[j];
+  return mul(unpack384(rr), fp_const(C_R3_INT));
+}
 
 // canonical plain value > (p-1)/2 (decompress.cu:107-135)
 __device__ __forceinline__ bool lex_largest(const Fp& a) {

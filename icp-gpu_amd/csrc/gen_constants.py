@@ -125,6 +125,7 @@ def main():
     arr("C_TO_INT", pow(2, 400, P), "2^400 mod p: mont(x, C_TO_INT) maps x*2^384 -> x*2^392")
     arr("C_TO_REF", pow(2, 384, P), "2^384 mod p: mont(x, C_TO_REF) maps x*2^392 -> x*2^384")
     arr("C_R2_INT", RI * RI % P, "R'^2 mod p: mont(v, C_R2_INT) maps plain v -> v*R'")
+    arr("C_R3_INT", RI ** 3 % P, "R'^3 mod p: mont(x^-1, C_R3_INT) maps the inverse of x = aR' to a^-1 R'")
     arr("FROB6_C1_1", to_int(g1[1]), "(1+u)^((p-1)/3) = (0, this)")
     arr("FROB6_C2_0", to_int(g2[0]), "(1+u)^(2(p-1)/3) = (this, 0)")
     arr("FROB12_0", to_int(g12[0]), "(1+u)^((p-1)/6) = (this, .)")

@@ -13,10 +13,12 @@ symbolically.  They follow the reference's formulas (gpu-crypto/cuda/field_sppar
 52-197, api.cu:37 for the line evaluation); the Frobenius maps are expressed with
 constants computed here from their closed forms ((1+u)^((p-1)/3) etc.).
 
-Term encoding (two bytes): a term byte whose bits 7..6 select the operand region
-(0 = A, 1 = B, 2 = T product scratch, 3 = K constants) and bits 5..0 the Fp index
-inside the region, then a signed coefficient byte (|c| <= 8).  Product record: dst
-(T index), na, nb, then na + nb terms.  Output record: dst index, n, then n terms.
+Term encoding (one byte): bits 7..6 select the operand region (0 = A, 1 = B, 2 = T
+product scratch, 3 = K constants), bits 5..0 the Fp index inside the region.  A
+linear combination is a count of positive terms, a count of negative terms, then the
+terms (a coefficient c appears as |c| copies of its term).  Product record: dst (T
+index), a_pos, a_neg, b_pos, b_neg, then the terms of A and of B.  Output record: dst
+index, n_pos, n_neg, then the terms.
 """
 import os
 import sys
@@ -405,6 +407,18 @@ def op_inv12():
     return R, outs_of(F12(f6mul(R, a.c0, ti), -f6mul(R, a.c1, ti)))
 
 
+def op_scale2():
+    # both pairs at once: A = (line1 (6), line2 (6)), B = (P1.x, P1.y, P2.x, P2.y);
+    # outputs (c2, c1*Px, c0*Py) of pair 1 then of pair 2 (12 Fp)
+    R = Rec()
+    out = []
+    for pair in range(2):
+        c0, c1, c2 = f2(A, 3 * pair), f2(A, 3 * pair + 1), f2(A, 3 * pair + 2)
+        x, y = fp(B, 2 * pair), fp(B, 2 * pair + 1)
+        out += outs_of(c2) + outs_of(f2mul_fp(R, c1, x)) + outs_of(f2mul_fp(R, c0, y))
+    return R, out
+
+
 OPS = [
     ("OP_MUL", op_mul),
     ("OP_SQR", op_sqr),
@@ -424,6 +438,7 @@ OPS = [
     ("OP_INV2", op_inv2),
     ("OP_INV6", op_inv6),
     ("OP_INV12", op_inv12),
+    ("OP_SCALE2", op_scale2),
 ]
 
 
@@ -433,12 +448,13 @@ def term_byte(region, idx):
 
 
 def encode_terms(lin):
-    """[(term byte, signed coefficient byte)...] in a canonical order"""
-    out = []
+    """(n_pos, n_neg, term bytes): positive terms then negative terms, a coefficient c
+    expanded into |c| copies of its term"""
+    pos, neg = [], []
     for (region, idx), c in sorted(lin.t.items()):
         assert -8 <= c <= 8, c
-        out += [term_byte(region, idx), c & 0xff]
-    return out
+        (pos if c > 0 else neg).extend([term_byte(region, idx)] * abs(c))
+    return len(pos), len(neg), pos + neg
 
 
 def limbs28(v):
@@ -452,18 +468,33 @@ def main():
     max_terms = 0
     for name, fn in OPS:
         R, outs = fn()
+        # products are independent: order them by operand term count so that the lanes
+        # of one round (products j = r*G .. r*G+G-1) do similar amounts of gathering;
+        # outputs carry their destination index, so they can be ordered the same way
+        order = sorted(range(len(R.products)),
+                       key=lambda j: sum(abs(c) for c in R.products[j][0].t.values()) +
+                       sum(abs(c) for c in R.products[j][1].t.values()))
+        remap = {old: new for new, old in enumerate(order)}
+        R.products = [R.products[j] for j in order]
+
+        def ren(lin):
+            return Lin({((T, remap[i]) if r == T else (r, i)): c for (r, i), c in lin.t.items()})
+        outs = [ren(o) for o in outs]
+        out_order = sorted(range(len(outs)), key=lambda i: sum(abs(c) for c in outs[i].t.values()))
         p_start, o_start = [], []
         for j, (a, b) in enumerate(R.products):
-            ea, eb = encode_terms(a), encode_terms(b)
-            max_terms = max(max_terms, len(ea) // 2, len(eb) // 2)
+            ap, an, ea = encode_terms(a)
+            bp, bn, eb = encode_terms(b)
+            max_terms = max(max_terms, len(ea), len(eb))
             p_start.append(len(code))
-            code += [j, len(ea) // 2, len(eb) // 2] + ea + eb
-        for i, o in enumerate(outs):
+            code += [j, ap, an, bp, bn] + ea + eb
+        for i in out_order:
             # output i is written to index i of the destination register
-            eo = encode_terms(o)
-            max_terms = max(max_terms, len(eo) // 2)
+            o = outs[i]
+            op_, on, eo = encode_terms(o)
+            max_terms = max(max_terms, len(eo))
             o_start.append(len(code))
-            code += [i, len(eo) // 2] + eo
+            code += [i, op_, on] + eo
         # hazard check: an output that reads input A index k other than its own index
         # would see a value overwritten in place; the kernels write outputs to a
         # distinct register whenever this is flagged.
@@ -491,10 +522,13 @@ def main():
     lines.append("constexpr uint16_t START_OFF[NUM_OPS] = {%s};  // index into START: products then outputs" %
                  ", ".join(str(o) for o in offs))
     lines.append("constexpr int START_LEN = %d;" % len(starts))
-    lines.append("__device__ const uint16_t START[%d] = {%s};" % (len(starts), ", ".join(map(str, starts))))
+    starts_p = starts + [0] * (len(starts) % 2)
+    lines.append("__device__ __align__(4) const uint16_t START[%d] = {%s};" % (len(starts_p), ", ".join(map(str, starts_p))))
     lines.append("constexpr int CODE_LEN = %d;" % len(code))
     body = ", ".join(str(c) for c in code)
-    lines.append("__device__ const uint8_t CODE[%d] = {%s};" % (len(code), body))
+    pad = (-len(code)) % 4
+    body = ", ".join(str(c) for c in code + [0] * pad)
+    lines.append("__device__ __align__(4) const uint8_t CODE[%d] = {%s};" % (len(code) + pad, body))
     lines.append("// constants (internal Montgomery form, 28-bit limbs)")
     kvals = []
     for v in CONSTS.vals:

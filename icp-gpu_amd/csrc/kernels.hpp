@@ -23,12 +23,18 @@ int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
                      int block, hipStream_t s);
 
-// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 16, 32;
-// final exponentiation: 16, 32).  Return -1 for an unsupported G or a launch error.
+// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13,
+// 16, 32; final exponentiation: 12, 16, 18, 32).  Return -1 for an unsupported G or a launch error.
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                       const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
                       hipStream_t s);
-int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, int32_t* result, uint64_t* gt, int M,
-                         hipStream_t s);
+int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
+                         uint64_t* gt, int M, hipStream_t s);
+
+// G1 decoding in two parts (kernels_g1.hip): decode (flags, range, square root, sign)
+// and the subgroup test, which may run concurrently with the Miller loop.
+int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s);
+int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s);
+int launch_merge_status(int32_t* status, const int32_t* sub_status, int M, hipStream_t s);
 
 }  // namespace bls

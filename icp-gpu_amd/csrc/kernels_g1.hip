@@ -1,0 +1,116 @@
+// G1 signature decoding, split in two kernels so that the subgroup test can run
+// concurrently with the Miller loop (which needs only the decoded coordinates):
+//   k_g1_decode   flags, x < p, y = sqrt(x^3 + 4) with the sign choice -> point + status
+//   k_g1_subgroup [x^2] P == (beta x, -y) on decoded points -> subgroup status
+// Semantics as ic_bls12_381 G1Affine::from_compressed; the reference GPU decoder
+// (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the decoding steps
+// without any of the validity checks.  Compiled with the field multiplier inlined.
+#include <hip/hip_runtime.h>
+
+#include "curve.hpp"
+#include "kernels.hpp"
+
+namespace bls {
+
+__device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < NL; j++) r.l[j] = base[(size_t)(w0 + j) * M + i];
+  return r;
+}
+__device__ __forceinline__ void st_soa(i32* base, int w0, int M, int i, const Fp& v) {
+#pragma unroll
+  for (int j = 0; j < NL; j++) base[(size_t)(w0 + j) * M + i] = v.l[j];
+}
+
+__global__ __launch_bounds__(64) void k_g1_decode(const uint8_t* __restrict__ in, i32* __restrict__ pts,
+                                                  i32* __restrict__ status, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  u32 be[12];
+  const uint8_t* b = in + (size_t)i * 48;
+#pragma unroll
+  for (int j = 0; j < 12; j++)
+    be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
+  Aff<Fp> P;
+  const int32_t st = g1_decompress(be, P, false);
+  st_soa(pts, 0, M, i, P.x);
+  st_soa(pts, NL, M, i, P.y);
+  status[i] = st;
+}
+
+// Jacobian + Jacobian (add-2007-bl), a = 0, with the doubling and identity cases.
+__device__ __forceinline__ Jac<Fp> jadd(const Jac<Fp>& p, const Jac<Fp>& q) {
+  if (is_zero(p.z)) return q;
+  if (is_zero(q.z)) return p;
+  Fp z1z1 = sqr(p.z), z2z2 = sqr(q.z);
+  Fp u1 = mul(p.x, z2z2), u2 = mul(q.x, z1z1);
+  Fp s1 = mul(p.y, mul(q.z, z2z2)), s2 = mul(q.y, mul(p.z, z1z1));
+  Fp h = norm(sub(u2, u1));
+  Fp r = norm(dbl(sub(s2, s1)));
+  if (is_zero(h)) {
+    if (is_zero(r)) return jdbl(p);
+    return {fp_one(), fp_one(), fp_zero()};
+  }
+  Fp i = sqr(norm(dbl(h)));
+  Fp j = mul(h, i);
+  Fp v = mul(u1, i);
+  Jac<Fp> o;
+  o.x = norm(sub(sub(sub(sqr(r), j), v), v));
+  o.y = norm(sub(mul(r, sub(v, o.x)), dbl(mul(s1, j))));
+  o.z = mul(norm(sub(sub(sqr(carry(add(p.z, q.z))), z1z1), z2z2)), h);
+  return o;
+}
+__device__ __forceinline__ Jac<Fp> mul_absx_jac(const Jac<Fp>& q) {
+  Jac<Fp> acc = q;
+  for (int i = 62; i >= 0; i--) {
+    acc = jdbl(acc);
+    if ((BLS_X >> i) & 1) acc = jadd(acc, q);
+  }
+  return acc;
+}
+
+// Points whose decode status is OK get sub_status 0 (in G1) or DEC_NOT_IN_SUBGROUP.
+// [x^2]P is computed as [|x|]([|x|]P) without leaving Jacobian coordinates, and compared
+// with (beta x, -y) projectively: X == beta x Z^2 and Y == -y Z^3.
+__global__ __launch_bounds__(64) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
+                                                    i32* __restrict__ sub_status, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  if (status[i] != DEC_OK) {
+    sub_status[i] = DEC_OK;
+    return;
+  }
+  const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
+  Jac<Fp> p1 = {x, y, fp_one()};
+  Jac<Fp> a = mul_absx_jac(p1);
+  Jac<Fp> b = mul_absx_jac(a);
+  bool in = false;
+  if (!is_zero(b.z)) {
+    Fp z2 = sqr(b.z);
+    in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
+  }
+  sub_status[i] = in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
+}
+
+// status := sub_status where status is OK (for entry points that report decoding alone)
+__global__ void k_merge_status(i32* __restrict__ status, const i32* __restrict__ sub_status, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  if (status[i] == DEC_OK && sub_status[i] != DEC_OK) status[i] = sub_status[i];
+}
+
+int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s) {
+  hipLaunchKernelGGL(k_g1_decode, dim3((M + 63) / 64), dim3(64), 0, s, in, pts, status, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s) {
+  hipLaunchKernelGGL(k_g1_subgroup, dim3((M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_merge_status(int32_t* status, const int32_t* sub_status, int M, hipStream_t s) {
+  hipLaunchKernelGGL(k_merge_status, dim3((M + 255) / 256), dim3(256), 0, s, status, sub_status, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace bls

@@ -18,10 +18,14 @@ namespace bls {
 
 constexpr int SLOT = 16;
 
-__device__ __forceinline__ Fp lds_ld(const i32* p) {
-  const int4* q = reinterpret_cast<const int4*>(p);
+// All LDS of these kernels is one dynamic array; device functions address it by word
+// offset so that they stay in the LDS address space across calls.
+extern __shared__ __align__(16) i32 g_lds[];
+
+__device__ __forceinline__ Fp lds_ld(int off) {
+  const int4* q = reinterpret_cast<const int4*>(g_lds + off);
   int4 a = q[0], b = q[1], c = q[2];
-  int2 d = reinterpret_cast<const int2*>(p)[6];
+  int2 d = reinterpret_cast<const int2*>(g_lds + off)[6];
   Fp r;
   r.l[0] = a.x; r.l[1] = a.y; r.l[2] = a.z; r.l[3] = a.w;
   r.l[4] = b.x; r.l[5] = b.y; r.l[6] = b.z; r.l[7] = b.w;
@@ -29,12 +33,12 @@ __device__ __forceinline__ Fp lds_ld(const i32* p) {
   r.l[12] = d.x; r.l[13] = d.y;
   return r;
 }
-__device__ __forceinline__ void lds_st(i32* p, const Fp& v) {
-  int4* q = reinterpret_cast<int4*>(p);
+__device__ __forceinline__ void lds_st(int off, const Fp& v) {
+  int4* q = reinterpret_cast<int4*>(g_lds + off);
   q[0] = make_int4(v.l[0], v.l[1], v.l[2], v.l[3]);
   q[1] = make_int4(v.l[4], v.l[5], v.l[6], v.l[7]);
   q[2] = make_int4(v.l[8], v.l[9], v.l[10], v.l[11]);
-  reinterpret_cast<int2*>(p)[6] = make_int2(v.l[12], v.l[13]);
+  reinterpret_cast<int2*>(g_lds + off)[6] = make_int2(v.l[12], v.l[13]);
 }
 
 __device__ __forceinline__ void st_ref_words(u64* p, const Fp& a) {
@@ -44,70 +48,118 @@ __device__ __forceinline__ void st_ref_words(u64* p, const Fp& a) {
   for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
 }
 
+// word offsets of the four operand regions of a schedule: A, B, T (products), K (constants)
 struct Regions {
-  i32* r[4];  // A, B, T (product scratch), K (constants)
+  int a, b, t, k;
+  __device__ __forceinline__ int base(int reg) const { return reg == 0 ? a : reg == 1 ? b : reg == 2 ? t : k; }
 };
 
-// sum_t c_t * x_t over encoded terms.  Every slot value has |limb| < 2^28, and
-// |c_t| <= 7, so a carry pass whenever the accumulated weight sum |c| would exceed 7
-// keeps every limb below 2^31.
-__device__ __forceinline__ Fp gather(const uint8_t* d, int n, const Regions& R) {
-  Fp acc = fp_zero();
-  int w = 0;
-  for (int t = 0; t < n; t++) {
-    const int tb = d[2 * t];
-    const int c = (int8_t)d[2 * t + 1];
-    const int reg = tb >> 6;
-    const i32* src = reg == 0 ? R.r[0] : reg == 1 ? R.r[1] : reg == 2 ? R.r[2] : R.r[3];
-    const Fp x = lds_ld(src + (tb & 63) * SLOT);
-    const int ac = c < 0 ? -c : c;
-    if (w + ac > 7) {
+// The schedule tables are copied into LDS at kernel start (byte offset SCHED_B of
+// g_lds for CODE, START after it), so that term decoding costs LDS latency only.
+constexpr int CODE_WORDS = (sched::CODE_LEN + 3) / 4;
+constexpr int START_WORDS = (sched::START_LEN + 1) / 2;
+constexpr int SCHED_WORDS = CODE_WORDS + START_WORDS;
+
+__device__ __forceinline__ void load_schedule(int sched_w) {
+  const uint32_t* code = reinterpret_cast<const uint32_t*>(sched::CODE);
+  const uint32_t* start = reinterpret_cast<const uint32_t*>(sched::START);
+  for (int w = threadIdx.x; w < SCHED_WORDS; w += blockDim.x)
+    g_lds[sched_w + w] = (i32)(w < CODE_WORDS ? code[w] : start[w - CODE_WORDS]);
+}
+__device__ __forceinline__ int code_byte(int sched_w, int idx) {
+  return reinterpret_cast<const uint8_t*>(g_lds + sched_w)[idx];
+}
+__device__ __forceinline__ int start_of(int sched_w, int idx) {
+  return reinterpret_cast<const uint16_t*>(g_lds + sched_w + CODE_WORDS)[idx];
+}
+
+// acc +/- sum of n terms starting at code byte `at`, in chunks of up to 6 terms whose
+// loads are all issued before their additions.  Every slot value has |limb| < 2^28 and a
+// carry pass whenever the accumulated weight would pass 7 keeps every limb below 2^31.
+template <bool SUB>
+__device__ __forceinline__ void gather_sign(Fp& acc, int& w, int sched_w, int at, int n, const Regions& R) {
+  for (int t = 0; t < n; t += 6) {
+    const int k = n - t < 6 ? n - t : 6;
+    Fp x[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      if (j < k) {
+        const int tb = code_byte(sched_w, at + t + j);
+        x[j] = lds_ld(R.base(tb >> 6) + (tb & 63) * SLOT);
+      }
+    }
+    if (w + k > 7) {
       acc = carry(acc);
       w = 1;
     }
 #pragma unroll
-    for (int i = 0; i < NL; i++) acc.l[i] += c * x.l[i];
-    w += ac;
+    for (int j = 0; j < 6; j++) {
+      if (j < k) {
+#pragma unroll
+        for (int i = 0; i < NL; i++) acc.l[i] = SUB ? acc.l[i] - x[j].l[i] : acc.l[i] + x[j].l[i];
+      }
+    }
+    w += k;
   }
+}
+__device__ __forceinline__ Fp gather(int sched_w, int at, int npos, int nneg, const Regions& R) {
+  Fp acc = fp_zero();
+  int w = 0;
+  gather_sign<false>(acc, w, sched_w, at, npos, R);
+  gather_sign<true>(acc, w, sched_w, at + npos, nneg, R);
   return acc;
 }
 
 // One schedule for the item of this lane's group.  Outputs go to D[0..NOUT); D may
 // alias A (every schedule used in place reads no other input component in its outputs).
-template <int G>
-__device__ __forceinline__ void run_op(int op, int sub, bool active, i32* A, i32* B, i32* D, i32* T, i32* K) {
-  const Regions R{{A, B, T, K}};
+// Not inlined: one copy of the multiplier serves every call site.
+__device__ __noinline__ void run_op_impl(int G, int op, int sub, bool active, int A, int B, int D, int T, int K,
+                                         int sched_w) {
+  const Regions R{A, B, T, K};
   const int np = sched::NPROD[op], no = sched::NOUT[op], so = sched::START_OFF[op];
   if (active) {
     for (int j = sub; j < np; j += G) {
-      const uint8_t* d = sched::CODE + sched::START[so + j];
-      const int dst = d[0], na = d[1], nb = d[2];
-      const Fp a = carry(gather(d + 3, na, R));
-      const Fp b = carry(gather(d + 3 + 2 * na, nb, R));
+      const int d = start_of(sched_w, so + j);
+      const int dst = code_byte(sched_w, d), ap = code_byte(sched_w, d + 1), an = code_byte(sched_w, d + 2);
+      const int bp = code_byte(sched_w, d + 3), bn = code_byte(sched_w, d + 4);
+      const Fp a = carry(gather(sched_w, d + 5, ap, an, R));
+      const Fp b = carry(gather(sched_w, d + 5 + ap + an, bp, bn, R));
       lds_st(T + dst * SLOT, mul(a, b));
     }
   }
   __syncthreads();
   if (active) {
     for (int o = sub; o < no; o += G) {
-      const uint8_t* d = sched::CODE + sched::START[so + np + o];
-      lds_st(D + d[0] * SLOT, norm(gather(d + 2, d[1], R)));
+      const int d = start_of(sched_w, so + np + o);
+      const int dst = code_byte(sched_w, d), np_ = code_byte(sched_w, d + 1), nn = code_byte(sched_w, d + 2);
+      lds_st(D + dst * SLOT, norm(gather(sched_w, d + 3, np_, nn, R)));
     }
   }
   __syncthreads();
 }
+// the schedule copy sits at the end of the dynamic LDS; each kernel passes its offset
+// through K: the constants region is followed by the schedule (see *_lds_bytes)
+template <int G>
+__device__ __forceinline__ void run_op(int op, int sub, bool active, int A, int B, int D, int T, int K) {
+  run_op_impl(G, op, sub, active, A, B, D, T, K, K + sched::NUM_CONSTS * SLOT);
+}
 
-__device__ __forceinline__ void load_consts(i32* K) {
+__device__ __forceinline__ void load_consts(int K) {
   for (int w = threadIdx.x; w < sched::NUM_CONSTS * NL; w += blockDim.x) {
     const int c = w / NL, l = w % NL;
-    K[c * SLOT + l] = sched::KCONST[c][l];
+    g_lds[K + c * SLOT + l] = sched::KCONST[c][l];
   }
 }
 
 // ---------------- Miller loop ----------------
 // slots per item: F 0..11 | L1 12..17 | L2 18..23 | P 24..27 (P1.x P1.y P2.x P2.y) |
-// S1 28..33 | S2 34..39 | T 40..78
+// S1 28..33 | S2 34..39 | T 40..78; the constants follow the last item.
 constexpr int ML_SLOTS = 79;
+
+template <int G>
+constexpr size_t miller_lds_bytes() {
+  return ((size_t)((64 / G) * ML_SLOTS + sched::NUM_CONSTS) * SLOT + SCHED_WORDS) * 4;
+}
 
 template <int G>
 __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
@@ -116,17 +168,17 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
                                                    const i32* __restrict__ pk_soa, i32* __restrict__ status,
                                                    i32* __restrict__ fout, int M) {
   constexpr int IPW = 64 / G;
-  __shared__ __align__(16) i32 lds[IPW * ML_SLOTS * SLOT + sched::NUM_CONSTS * SLOT];
   const int lid = threadIdx.x / G, sub = threadIdx.x % G;
   const bool in_wg = lid < IPW;
   const int item = blockIdx.x * IPW + lid;
   const bool valid = in_wg && item < M;
   const int it = valid ? item : 0;
-  i32* base = lds + (in_wg ? lid : 0) * ML_SLOTS * SLOT;
-  i32 *F = base, *L1 = base + 12 * SLOT, *L2 = base + 18 * SLOT, *P = base + 24 * SLOT;
-  i32 *S1 = base + 28 * SLOT, *S2 = base + 34 * SLOT, *T = base + 40 * SLOT;
-  i32* K = lds + IPW * ML_SLOTS * SLOT;
+  const int base = (in_wg ? lid : 0) * ML_SLOTS * SLOT;
+  const int F = base, L1 = base + 12 * SLOT, L2 = base + 18 * SLOT, P = base + 24 * SLOT;
+  const int S1 = base + 28 * SLOT, S2 = base + 34 * SLOT, T = base + 40 * SLOT;
+  const int K = IPW * ML_SLOTS * SLOT;
   load_consts(K);
+  load_schedule(K + sched::NUM_CONSTS * SLOT);
 
   bool skip_all = !valid, skip1 = false;
   int kidx = 0;
@@ -151,14 +203,15 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
     for (int w = sub; w < 4 * NL; w += G) {
       const int e = w / NL, l = w % NL;
       const i32* src = e < 2 ? sig : nhm;
-      P[e * SLOT + l] = src[(size_t)((e & 1) * NL + l) * M + it];
+      g_lds[P + e * SLOT + l] = src[(size_t)((e & 1) * NL + l) * M + it];
     }
     for (int w = sub; w < 12 * NL; w += G) {
       const int e = w / NL, l = w % NL;
-      F[e * SLOT + l] = 0;
+      g_lds[F + e * SLOT + l] = 0;
     }
-    if (sub == 0) lds_st(F, fp_one());
   }
+  __syncthreads();
+  if (in_wg && sub == 0) lds_st(F, fp_one());
   const i32* pk = key_idx ? keys + (size_t)kidx * KEY_TABLE_WORDS : nullptr;
   __syncthreads();
 
@@ -171,12 +224,11 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
         i32 v;
         if (which == 0) v = gen_lines[ci * 84 + ww];
         else v = pk ? pk[ci * 84 + ww] : pk_soa[(size_t)(ci * 84 + ww) * M + it];
-        (which == 0 ? L1 : L2)[e * SLOT + l] = v;
+        g_lds[(which == 0 ? L1 : L2) + e * SLOT + l] = v;
       }
     }
     __syncthreads();
-    run_op<G>(sched::OP_SCALE, sub, act1, L1, P, S1, T, K);
-    run_op<G>(sched::OP_SCALE, sub, act, L2, P + 2 * SLOT, S2, T, K);
+    run_op<G>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);  // S1 | S2 (both pairs)
     run_op<G>(sched::OP_ELL, sub, act1, F, S1, F, T, K);
     run_op<G>(sched::OP_ELL, sub, act, F, S2, F, T, K);
   };
@@ -184,98 +236,128 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
   for (int b = 61; b >= 0; b--) {
     step(ci++);
     if (miller_bit(b)) step(ci++);
-    run_op<G>(sched::OP_SQR, sub, act, F, nullptr, F, T, K);
+    run_op<G>(sched::OP_SQR, sub, act, F, 0, F, T, K);
   }
   step(ci);
-  run_op<G>(sched::OP_CONJ, sub, act, F, nullptr, F, T, K);
+  run_op<G>(sched::OP_CONJ, sub, act, F, 0, F, T, K);
   if (valid && act) {
     for (int w = sub; w < 12 * NL; w += G) {
       const int e = w / NL, l = w % NL;
-      fout[(size_t)w * M + item] = F[e * SLOT + l];
+      fout[(size_t)w * M + item] = g_lds[F + e * SLOT + l];
     }
   }
 }
 
 // ---------------- final exponentiation ----------------
-// slots per item: R0..R5 (12 each) 0..71 | T 72..125
+// slots per item: R0..R5 (12 each) 0..71 | T 72..125; then the constants, then the
+// per-item identity flags (12 words per item).
 constexpr int FE_SLOTS = 126;
 
 template <int G>
-__device__ __forceinline__ void cyc_exp_grp(int sub, bool act, i32* src, i32* dst, i32* T, i32* K) {
-  run_op<G>(sched::OP_COPY, sub, act, src, nullptr, dst, T, K);
+constexpr size_t final_exp_lds_bytes() {
+  return ((size_t)((64 / G) * FE_SLOTS + sched::NUM_CONSTS) * SLOT + SCHED_WORDS + (64 / G) * 12) * 4;
+}
+
+template <int G>
+__device__ __forceinline__ void cyc_exp_grp(int sub, bool act, int src, int dst, int T, int K) {
+  run_op<G>(sched::OP_COPY, sub, act, src, 0, dst, T, K);
   for (int i = 62; i >= 0; i--) {
-    run_op<G>(sched::OP_CYC_SQR, sub, act, dst, nullptr, dst, T, K);
+    run_op<G>(sched::OP_CYC_SQR, sub, act, dst, 0, dst, T, K);
     if ((BLS_X >> i) & 1) run_op<G>(sched::OP_MUL, sub, act, dst, src, dst, T, K);
   }
-  run_op<G>(sched::OP_CONJ, sub, act, dst, nullptr, dst, T, K);
+  run_op<G>(sched::OP_CONJ, sub, act, dst, 0, dst, T, K);
 }
 
 template <int G>
 __global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
-                                                      int32_t* __restrict__ result, u64* __restrict__ gt_out, int M) {
+                                                      const i32* __restrict__ sub_status, int32_t* __restrict__ result,
+                                                      u64* __restrict__ gt_out, int M) {
   constexpr int IPW = 64 / G;
-  __shared__ __align__(16) i32 lds[IPW * FE_SLOTS * SLOT + sched::NUM_CONSTS * SLOT];
-  __shared__ int flags[IPW][12];
   const int lid = threadIdx.x / G, sub = threadIdx.x % G;
   const bool in_wg = lid < IPW;
   const int item = blockIdx.x * IPW + lid;
   const bool valid = in_wg && item < M;
   const int it = valid ? item : 0;
-  i32* base = lds + (in_wg ? lid : 0) * FE_SLOTS * SLOT;
-  i32 *R0 = base, *R1 = base + 12 * SLOT, *R2 = base + 24 * SLOT, *R3 = base + 36 * SLOT;
-  i32 *R4 = base + 48 * SLOT, *R5 = base + 60 * SLOT, *T = base + 72 * SLOT;
-  i32* K = lds + IPW * FE_SLOTS * SLOT;
+  const int base = (in_wg ? lid : 0) * FE_SLOTS * SLOT;
+  const int R0 = base, R1 = base + 12 * SLOT, R2 = base + 24 * SLOT, R3 = base + 36 * SLOT;
+  const int R4 = base + 48 * SLOT, R5 = base + 60 * SLOT, T = base + 72 * SLOT;
+  const int K = IPW * FE_SLOTS * SLOT;
+  const int FLAGS = K + sched::NUM_CONSTS * SLOT + SCHED_WORDS + (in_wg ? lid : 0) * 12;
   load_consts(K);
+  load_schedule(K + sched::NUM_CONSTS * SLOT);
   int32_t st = DEC_OK;
   if (valid && status) st = status[it];
+  if (valid && sub_status && st == DEC_OK) st = sub_status[it];
   const bool act = valid && (st == DEC_OK || st == DEC_IDENTITY);
   if (in_wg) {
     for (int w = sub; w < 12 * NL; w += G) {
       const int e = w / NL, l = w % NL;
-      R0[e * SLOT + l] = act ? fin[(size_t)w * M + it] : 0;
+      g_lds[R0 + e * SLOT + l] = act ? fin[(size_t)w * M + it] : 0;
     }
   }
   __syncthreads();
 
   // easy part: f^((p^6-1)(p^2+1)) with f^-1 from the norm chain Fp12 -> Fp6 -> Fp2 -> Fp
-  run_op<G>(sched::OP_N12, sub, act, R0, nullptr, R1, T, K);           // R1[0..5] = c0^2 - v c1^2
-  run_op<G>(sched::OP_N6A, sub, act, R1, nullptr, R2, T, K);           // R2[0..5] = (t0, t1, t2)
-  run_op<G>(sched::OP_N6B, sub, act, R1, R2, R3, T, K);                // R3[0..1] = sc
-  run_op<G>(sched::OP_N2, sub, act, R3, nullptr, R3 + 2 * SLOT, T, K); // R3[2] = n
-  if (act && sub == 0) lds_st(R3 + 3 * SLOT, inv(lds_ld(R3 + 2 * SLOT)));
+  run_op<G>(sched::OP_N12, sub, act, R0, 0, R1, T, K);            // R1[0..5] = c0^2 - v c1^2
+  run_op<G>(sched::OP_N6A, sub, act, R1, 0, R2, T, K);            // R2[0..5] = (t0, t1, t2)
+  run_op<G>(sched::OP_N6B, sub, act, R1, R2, R3, T, K);           // R3[0..1] = sc
+  run_op<G>(sched::OP_N2, sub, act, R3, 0, R3 + 2 * SLOT, T, K);  // R3[2] = n
+  // one inversion per workgroup (Montgomery's trick over its IPW items): thread 0 forms
+  // the prefix products of the n's (inactive or zero entries count as 1), inverts the
+  // total by binary GCD and peels the individual inverses off into R3[3] of each item
+  if (threadIdx.x == 0) {
+    Fp pre[IPW];
+    Fp accp = fp_one();
+    for (int q = 0; q < IPW; q++) {
+      const int bq = q * FE_SLOTS * SLOT + 36 * SLOT;
+      Fp nq = lds_ld(bq + 2 * SLOT);
+      if (is_zero(nq)) nq = fp_one();
+      pre[q] = accp;
+      accp = mul(accp, nq);
+    }
+    Fp inv_all = inv_bgcd(accp);
+    for (int q = IPW - 1; q >= 0; q--) {
+      const int bq = q * FE_SLOTS * SLOT + 36 * SLOT;
+      Fp nq = lds_ld(bq + 2 * SLOT);
+      const bool z = is_zero(nq);
+      if (z) nq = fp_one();
+      lds_st(bq + 3 * SLOT, z ? fp_zero() : mul(inv_all, pre[q]));
+      inv_all = mul(inv_all, nq);
+    }
+  }
   __syncthreads();
   run_op<G>(sched::OP_INV2, sub, act, R3, R3 + 3 * SLOT, R3 + 4 * SLOT, T, K);  // R3[4..5] = sc^-1
   run_op<G>(sched::OP_INV6, sub, act, R2, R3 + 4 * SLOT, R4, T, K);             // R4[0..5] = t^-1
   run_op<G>(sched::OP_INV12, sub, act, R0, R4, R5, T, K);                       // R5 = f^-1
   run_op<G>(sched::OP_MUL_CONJA, sub, act, R0, R5, R1, T, K);                   // R1 = f^(p^6-1)
-  run_op<G>(sched::OP_FROB2, sub, act, R1, nullptr, R2, T, K);
+  run_op<G>(sched::OP_FROB2, sub, act, R1, 0, R2, T, K);
   run_op<G>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);                         // R2 = t2
 
   // hard part (api.cu:68-77).  Registers: F=R0 T1=R1 T2=R2 T3=R3 T4=R4 T0=R5
-  run_op<G>(sched::OP_COPY, sub, act, R2, nullptr, R0, T, K);       // f = t2
-  run_op<G>(sched::OP_CYC_SQR, sub, act, R2, nullptr, R1, T, K);
-  run_op<G>(sched::OP_CONJ, sub, act, R1, nullptr, R1, T, K);       // t1 = conj(cyc_sqr(t2))
-  cyc_exp_grp<G>(sub, act, R2, R3, T, K);                           // t3 = t2^x
-  run_op<G>(sched::OP_MUL, sub, act, R1, R3, R4, T, K);             // t4 = t1 t3
-  cyc_exp_grp<G>(sub, act, R4, R1, T, K);                           // t1 = t4^x
-  run_op<G>(sched::OP_CONJ, sub, act, R4, nullptr, R4, T, K);       // t4 = conj(t4)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);             // f *= t4
-  run_op<G>(sched::OP_CYC_SQR, sub, act, R3, nullptr, R4, T, K);    // t4 = cyc_sqr(t3)
-  cyc_exp_grp<G>(sub, act, R1, R5, T, K);                           // t0 = t1^x
-  run_op<G>(sched::OP_MUL, sub, act, R3, R5, R3, T, K);             // t3 *= t0
-  run_op<G>(sched::OP_FROB2, sub, act, R3, nullptr, R3, T, K);      // t3 = frob^2(t3)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);             // f *= t3
-  cyc_exp_grp<G>(sub, act, R5, R3, T, K);                           // tmp = t0^x
-  run_op<G>(sched::OP_MUL, sub, act, R4, R3, R4, T, K);             // t4 *= tmp
-  cyc_exp_grp<G>(sub, act, R4, R3, T, K);                           // tmp = t4^x
-  run_op<G>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);             // f *= tmp
-  run_op<G>(sched::OP_CONJ, sub, act, R2, nullptr, R5, T, K);       // conj(t2)
-  run_op<G>(sched::OP_MUL, sub, act, R4, R5, R4, T, K);             // t4 *= conj(t2)
-  run_op<G>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);             // t2 *= t1
-  run_op<G>(sched::OP_FROB3, sub, act, R2, nullptr, R2, T, K);      // t2 = frob^3(t2)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R2, R0, T, K);             // f *= t2
-  run_op<G>(sched::OP_FROB1, sub, act, R4, nullptr, R4, T, K);      // t4 = frob(t4)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);             // f *= t4
+  run_op<G>(sched::OP_COPY, sub, act, R2, 0, R0, T, K);       // f = t2
+  run_op<G>(sched::OP_CYC_SQR, sub, act, R2, 0, R1, T, K);
+  run_op<G>(sched::OP_CONJ, sub, act, R1, 0, R1, T, K);       // t1 = conj(cyc_sqr(t2))
+  cyc_exp_grp<G>(sub, act, R2, R3, T, K);                     // t3 = t2^x
+  run_op<G>(sched::OP_MUL, sub, act, R1, R3, R4, T, K);       // t4 = t1 t3
+  cyc_exp_grp<G>(sub, act, R4, R1, T, K);                     // t1 = t4^x
+  run_op<G>(sched::OP_CONJ, sub, act, R4, 0, R4, T, K);       // t4 = conj(t4)
+  run_op<G>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);       // f *= t4
+  run_op<G>(sched::OP_CYC_SQR, sub, act, R3, 0, R4, T, K);    // t4 = cyc_sqr(t3)
+  cyc_exp_grp<G>(sub, act, R1, R5, T, K);                     // t0 = t1^x
+  run_op<G>(sched::OP_MUL, sub, act, R3, R5, R3, T, K);       // t3 *= t0
+  run_op<G>(sched::OP_FROB2, sub, act, R3, 0, R3, T, K);      // t3 = frob^2(t3)
+  run_op<G>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);       // f *= t3
+  cyc_exp_grp<G>(sub, act, R5, R3, T, K);                     // tmp = t0^x
+  run_op<G>(sched::OP_MUL, sub, act, R4, R3, R4, T, K);       // t4 *= tmp
+  cyc_exp_grp<G>(sub, act, R4, R3, T, K);                     // tmp = t4^x
+  run_op<G>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);       // f *= tmp
+  run_op<G>(sched::OP_CONJ, sub, act, R2, 0, R5, T, K);       // conj(t2)
+  run_op<G>(sched::OP_MUL, sub, act, R4, R5, R4, T, K);       // t4 *= conj(t2)
+  run_op<G>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);       // t2 *= t1
+  run_op<G>(sched::OP_FROB3, sub, act, R2, 0, R2, T, K);      // t2 = frob^3(t2)
+  run_op<G>(sched::OP_MUL, sub, act, R0, R2, R0, T, K);       // f *= t2
+  run_op<G>(sched::OP_FROB1, sub, act, R4, 0, R4, T, K);      // t4 = frob(t4)
+  run_op<G>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);       // f *= t4
 
   // identity test and export: component c handled by lane c mod G
   if (in_wg) {
@@ -289,13 +371,13 @@ __global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fi
       } else if (valid && gt_out) {
         for (int j = 0; j < 6; j++) gt_out[(size_t)it * 72 + c * 6 + j] = 0;
       }
-      flags[lid][c] = ok;
+      g_lds[FLAGS + c] = ok;
     }
   }
   __syncthreads();
   if (valid && sub == 0) {
     int all = 1;
-    for (int k = 0; k < 12; k++) all &= flags[lid][k];
+    for (int k = 0; k < 12; k++) all &= g_lds[FLAGS + k];
     result[it] = act ? all : st;
   }
 }
@@ -306,14 +388,16 @@ static int launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen,
                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
                      hipStream_t s) {
   constexpr int IPW = 64 / G;
-  hipLaunchKernelGGL(k_miller_grp<G>, dim3((M + IPW - 1) / IPW), dim3(64), 0, s, sig, nhm, gen, keys, key_idx, nkeys,
-                     pk_soa, status, fout, M);
+  hipLaunchKernelGGL(k_miller_grp<G>, dim3((M + IPW - 1) / IPW), dim3(64), miller_lds_bytes<G>(), s, sig, nhm, gen,
+                     keys, key_idx, nkeys, pk_soa, status, fout, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int G>
-static int launch_fe(const int32_t* fin, const int32_t* status, int32_t* result, uint64_t* gt, int M, hipStream_t s) {
+static int launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
+                     uint64_t* gt, int M, hipStream_t s) {
   constexpr int IPW = 64 / G;
-  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3((M + IPW - 1) / IPW), dim3(64), 0, s, fin, status, result, gt, M);
+  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3((M + IPW - 1) / IPW), dim3(64), final_exp_lds_bytes<G>(), s, fin, status,
+                     sub_status, result, gt, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -322,16 +406,20 @@ int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32
                       hipStream_t s) {
   switch (G) {
     case 8: return launch_ml<8>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
+    case 12: return launch_ml<12>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
+    case 13: return launch_ml<13>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
     case 16: return launch_ml<16>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
     case 32: return launch_ml<32>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
     default: return -1;
   }
 }
-int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, int32_t* result, uint64_t* gt, int M,
-                         hipStream_t s) {
+int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
+                         uint64_t* gt, int M, hipStream_t s) {
   switch (G) {
-    case 16: return launch_fe<16>(fin, status, result, gt, M, s);
-    case 32: return launch_fe<32>(fin, status, result, gt, M, s);
+    case 12: return launch_fe<12>(fin, status, sub_status, result, gt, M, s);
+    case 16: return launch_fe<16>(fin, status, sub_status, result, gt, M, s);
+    case 18: return launch_fe<18>(fin, status, sub_status, result, gt, M, s);
+    case 32: return launch_fe<32>(fin, status, sub_status, result, gt, M, s);
     default: return -1;
   }
 }

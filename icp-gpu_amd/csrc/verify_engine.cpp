@@ -12,9 +12,12 @@ static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, 
                          a.block, s);
   return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_soa, status, a.fbuf, a.M, s);
 }
-static int final_exp(const EngineArgs& a, const int32_t* status, hipStream_t s) {
-  if (a.mode == 0) return launch_final_exp(a.fbuf, status, a.results, a.gt, a.M, a.block, s);
-  return launch_final_exp_grp(a.g_fe, a.fbuf, status, a.results, a.gt, a.M, s);
+static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* sub_status, hipStream_t s) {
+  if (a.mode == 0) {
+    if (sub_status && launch_merge_status(const_cast<int32_t*>(status), sub_status, a.M, s)) return -1;
+    return launch_final_exp(a.fbuf, status, a.results, a.gt, a.M, a.block, s);
+  }
+  return launch_final_exp_grp(a.g_fe, a.fbuf, status, sub_status, a.results, a.gt, a.M, s);
 }
 
 int engine_legacy(const EngineArgs& a, hipStream_t s) {
@@ -23,19 +26,36 @@ int engine_legacy(const EngineArgs& a, hipStream_t s) {
   if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, 0, s)) return -1;
   if (launch_lines_from_ref(a.pk_ref, a.pk_lines, a.M, 1, s)) return -1;
   if (miller(a, a.g2_lines, nullptr, nullptr, a.pk_lines, nullptr, s)) return -1;
-  return final_exp(a, nullptr, s);
+  return final_exp(a, nullptr, nullptr, s);
 }
 
 static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
   return a.ev ? (hipEventRecord(a.ev[k], s) == hipSuccess ? 0 : -1) : 0;
 }
 
+// decode -> fork: [subgroup test on s2] || [Miller loop on s] -> join -> final exp
 int engine_compressed(const EngineArgs& a, hipStream_t s) {
-  if (mark(a, 0, s) || launch_g1_decompress(a.comp, a.sig_pts, a.status, a.M, 1, s)) return -1;
+  if (mark(a, 0, s) || launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
   if (mark(a, 1, s) || launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
+  const bool fork = a.s2 && a.fork && a.join;
+  if (fork) {
+    if (hipEventRecord(a.fork, s) != hipSuccess || hipStreamWaitEvent(a.s2, a.fork, 0) != hipSuccess) return -1;
+    if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, a.s2)) return -1;
+    if (hipEventRecord(a.join, a.s2) != hipSuccess) return -1;
+  } else if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, s)) {
+    return -1;
+  }
   if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s)) return -1;
-  if (mark(a, 3, s) || final_exp(a, a.status, s)) return -1;
+  if (fork && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
+  if (mark(a, 3, s) || final_exp(a, a.status, a.sub_status, s)) return -1;
   return mark(a, 4, s);
+}
+
+int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s) {
+  if (launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
+  if (!check_subgroup) return 0;
+  if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, s)) return -1;
+  return launch_merge_status(a.status, a.sub_status, a.M, s);
 }
 
 }  // namespace bls

@@ -23,7 +23,8 @@ struct EngineArgs {
   int32_t* nhm_pts = nullptr;   // M x 28
   int32_t* g2_lines = nullptr;  // 5712 (legacy)
   int32_t* pk_lines = nullptr;  // M x 5712 (legacy)
-  int32_t* status = nullptr;    // M (fused)
+  int32_t* status = nullptr;    // M (fused): decode status
+  int32_t* sub_status = nullptr; // M (fused): subgroup status
   int32_t* fbuf = nullptr;      // M x 168 Miller values
   // outputs
   int32_t* results = nullptr;  // M
@@ -35,9 +36,14 @@ struct EngineArgs {
   int g_fe = 16;
   // optional timing: events recorded before each phase and at the end (5 events)
   hipEvent_t* ev = nullptr;
+  // second stream for the subgroup test, and the fork/join events (fused path)
+  hipStream_t s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 int engine_legacy(const EngineArgs& a, hipStream_t s);
 int engine_compressed(const EngineArgs& a, hipStream_t s);
+// decode (and optionally subgroup-check) a.comp into a.sig_pts / a.status
+int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s);
 
 }  // namespace bls

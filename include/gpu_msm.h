@@ -106,8 +106,9 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
                                     int M, void* stream);
 
 /* Engine selection: mode 0 = one thread per verification (thread-per-item kernels),
- * mode 1 = a group of G lanes of one wavefront per verification (g_miller in {8, 16, 32}
- * for the Miller loop, g_final_exp in {16, 32} for the final exponentiation).  The
+ * mode 1 = a group of G lanes of one wavefront per verification (g_miller in
+ * {8, 12, 13, 16, 32} for the Miller loop, g_final_exp in {12, 16, 18, 32} for the final
+ * exponentiation).  The
  * default is mode 1 with 16/16; the autotuner in the Python binding picks per batch size. */
 int blsgpu_set_engine(int mode, int g_miller, int g_final_exp);
 int blsgpu_get_engine(int* mode, int* g_miller, int* g_final_exp);
