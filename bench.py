@@ -9,6 +9,18 @@ are decompressed and validated on the device; -H(m) arrives in ic_bls12_381's
 Montgomery form; the 4096 per-key line tables are resident in HBM (registered once,
 outside the timed region), as are all inputs when the timed region starts.
 
+Beside that headline step the same run measures the other forms on their workloads:
+  batched   256 verify batches of geometric sizes (mean 128, [1, 2048]) over a 13-key
+            pool, 1024 messages each signed by a random 1..13 signers, 1/128 invalid:
+            the plain form (one kernel sequence per batch) against the batched form
+            (one graph-replayed sequence with queue-fed persistent kernels, autotuned);
+  at_scale  65536 verifications over 40 keys and 4096 messages, lognormal batch sizes
+            (median 64, sigma 1.2, [1, 8192]), batched form (and one plain pass);
+  serving   Poisson arrivals at 2k, 20k and 100k verifications/s, lognormal request
+            sizes (median 16, sigma 1.0, [1, 512]) over 40 keys, 1/256 invalid, under a
+            4 GB HBM budget: p50/p99/p999 latency and sustained throughput of the
+            continuous-batching server and of the plain one-request-per-step form.
+
 Prints one JSON line (see the keys below).  The CPU baseline is the test oracle (a C
 restatement of the reference's CPU path) timed on the host's cores on a bounded sample
 of the same batch; its verdicts are also compared with the GPU's on that sample.
@@ -42,10 +54,16 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--engine", default="1:16:16", help="mode:lanes_miller:lanes_final_exp")
+    ap.add_argument("--engine", default="1:12:16", help="mode:lanes_miller:lanes_final_exp")
     ap.add_argument("--sweep", action="store_true", help="time every engine configuration first")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
+    ap.add_argument("--forms", default="plain,batched,at_scale,serving",
+                    help="comma list of forms to measure beside the headline plain form")
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of each serving run (60 for the full runs)")
+    ap.add_argument("--serve-rates", default="2000,20000,100000")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner for the batched forms")
     return ap.parse_args()
 
 
@@ -116,6 +134,176 @@ def cpu_baseline(sig, nhm, pk, key_idx, gpu_res, budget_s):
     }
 
 
+def gen_items(wl, seed, sk, key_idx, msg_idx, cat):
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    n = len(key_idx)
+    sig = np.zeros((n, 48), np.uint8)
+    nhm = np.zeros((n, 12), np.uint64)
+    key_idx = np.ascontiguousarray(key_idx, np.int32)
+    msg_idx = np.ascontiguousarray(msg_idx, np.int64)
+    cat = np.ascontiguousarray(cat, np.int8)
+    if wl.blsgpu_wl_items_msg(ctypes.c_uint64(seed), n, p(sk), sk.shape[0], p(key_idx), p(msg_idx), p(cat), p(sig),
+                              p(nhm)):
+        raise RuntimeError("items: " + wl.blsgpu_wl_last_error().decode())
+    return sig, nhm
+
+
+def gen_keys(wl, seed, nkeys):
+    sk = np.zeros((nkeys, 32), np.uint8)
+    wl.blsgpu_wl_secret_keys(ctypes.c_uint64(seed), nkeys, sk.ctypes.data_as(ctypes.c_void_p))
+    pk = np.zeros((nkeys, 24), np.uint64)
+    if wl.blsgpu_wl_public_keys(sk.ctypes.data_as(ctypes.c_void_p), nkeys, pk.ctypes.data_as(ctypes.c_void_p)):
+        raise RuntimeError("public keys: " + wl.blsgpu_wl_last_error().decode())
+    return sk, pk
+
+
+def mixed_invalid(rng, n, every):
+    """categories: 1/every of the items invalid, cycling wrong-message, no-sqrt, off-subgroup"""
+    cat = np.zeros(n, np.int8)
+    bad = rng.choice(n, size=n // every, replace=False)
+    cat[bad] = 1 + (np.arange(len(bad)) % 3)
+    return cat
+
+
+def batched_workload(wl, seed):
+    """256 batches, geometric sizes (mean 128, [1, 2048]); 13 signers; 1024 messages each
+    signed by a random subset of 1..13 signers; items drawn from those signatures; 1/128 invalid"""
+    rng = np.random.default_rng(seed)
+    sk, pk = gen_keys(wl, seed, 13)
+    pool = []
+    for m in range(1024):
+        k = int(rng.integers(1, 14))
+        for s in rng.choice(13, size=k, replace=False):
+            pool.append((m, int(s)))
+    pool = np.array(pool)
+    sizes = np.clip(rng.geometric(1.0 / 128, size=256), 1, 2048)
+    n = int(sizes.sum())
+    pick = rng.integers(0, len(pool), n)
+    msg_idx, key_idx = pool[pick, 0], pool[pick, 1]
+    cat = mixed_invalid(rng, n, 128)
+    sig, nhm = gen_items(wl, seed, sk, key_idx, msg_idx, cat)
+    return dict(sk=sk, pk=pk, sizes=sizes, key_idx=key_idx, cat=cat, sig=sig, nhm=nhm, signatures=len(pool))
+
+
+def at_scale_workload(wl, seed, total=65536):
+    """40 keys, 4096 messages, lognormal batch sizes (median 64, sigma 1.2, [1, 8192])"""
+    rng = np.random.default_rng(seed)
+    sk, pk = gen_keys(wl, seed, 40)
+    sizes = []
+    while sum(sizes) < total:
+        sizes.append(int(np.clip(round(np.exp(np.log(64) + 1.2 * rng.normal())), 1, 8192)))
+    sizes[-1] -= sum(sizes) - total
+    sizes = np.array([x for x in sizes if x > 0])
+    key_idx = rng.integers(0, 40, total)
+    msg_idx = rng.integers(0, 4096, total)
+    cat = mixed_invalid(rng, total, 128)
+    sig, nhm = gen_items(wl, seed, sk, key_idx, msg_idx, cat)
+    return dict(sk=sk, pk=pk, sizes=sizes, key_idx=key_idx, cat=cat, sig=sig, nhm=nhm)
+
+
+def serving_pool(wl, seed, sk, n=65536):
+    """fresh random messages over the given keys, 1/256 invalid"""
+    rng = np.random.default_rng(seed)
+    key_idx = rng.integers(0, sk.shape[0], n)
+    msg_idx = 5_000_000 + np.arange(n)
+    cat = mixed_invalid(rng, n, 256)
+    sig, nhm = gen_items(wl, seed, sk, key_idx, msg_idx, cat)
+    return dict(key_idx=key_idx, cat=cat, sig=sig, nhm=nhm)
+
+
+def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune):
+    """plain form (one kernel sequence per batch) vs batched form (one graph-replayed
+    sequence, queue-fed persistent kernels) on the same device-resident batches"""
+    dev = torch.device("cuda:0")
+    sizes = np.asarray(w["sizes"], np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n, nb = int(off[-1]), len(sizes)
+    d_sig = torch.from_numpy(w["sig"].reshape(-1)).to(dev)
+    d_nhm = torch.from_numpy(w["nhm"].view(np.int64).reshape(-1)).to(dev)
+    d_key = torch.from_numpy(handles[w["key_idx"]].astype(np.int32)).to(dev)
+    d_res_b = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_res_p = torch.zeros(n, dtype=torch.int32, device=dev)
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    offp = off.ctypes.data_as(ctypes.c_void_p)
+    tuned = None
+    if tune:
+        cfg = np.zeros(4, np.int32)
+        ms = np.zeros(2, np.float64)
+        t_t = time.perf_counter()
+        if lib.blsgpu_autotune_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(), n, sp,
+                                      cfg.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p)):
+            raise RuntimeError("autotune: " + blsgpu.last_error())
+        tuned = {"lanes_miller": int(cfg[0]), "lanes_final_exp": int(cfg[1]), "wg_per_cu_miller": int(cfg[2]),
+                 "wg_per_cu_final_exp": int(cfg[3]), "miller_ms": float(ms[0]), "final_exp_ms": float(ms[1]),
+                 "tuning_s": round(time.perf_counter() - t_t, 2)}
+        log("autotuned for %d items: %s" % (n, tuned))
+
+    def batched():
+        if lib.blsgpu_verify_batches_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(), offp, nb,
+                                            d_res_b.data_ptr(), None, sp):
+            raise RuntimeError(blsgpu.last_error())
+
+    def plain():
+        for b in range(nb):
+            if not sizes[b]:
+                continue
+            o = int(off[b])
+            if lib.blsgpu_verify_compressed_device(d_sig.data_ptr() + 48 * o, d_nhm.data_ptr() + 96 * o,
+                                                   d_key.data_ptr() + 4 * o, d_res_p.data_ptr() + 4 * o, None,
+                                                   int(sizes[b]), sp):
+                raise RuntimeError(blsgpu.last_error())
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    batched()
+    batched()
+    plain()
+    ms_b = timed(batched, steps)
+    ms_p = timed(plain, plain_steps)
+    want = expected_codes(w["cat"])
+    rb, rp = d_res_b.cpu().numpy(), d_res_p.cpu().numpy()
+    return {
+        "verifications": n, "batches": nb, "batch_size_min": int(sizes.min()), "batch_size_max": int(sizes.max()),
+        "batched_verif_per_s": n / (ms_b * 1e-3), "plain_verif_per_s": n / (ms_p * 1e-3),
+        "batched_ms": ms_b, "plain_ms": ms_p, "speedup": ms_p / ms_b,
+        "steps_batched": steps, "steps_plain": plain_steps,
+        "engine": tuned or dict(zip(("lanes_miller", "lanes_final_exp", "wg_per_cu_miller", "wg_per_cu_final_exp",
+                                     "graph"), blsgpu.get_batched_engine(n))),
+        "graphs": dict(zip(("instantiated", "launches"), blsgpu.batched_stats())),
+        "verdicts_as_expected": bool((rb == want).all() and (rp == want).all()),
+        "batched_equals_plain": bool((rb == rp).all()),
+    }
+
+
+def measure_serving(blsgpu, pool, handles, rates, seconds, seed):
+    """continuous-batching server vs the plain one-request-per-step form, per arrival rate"""
+    keys = handles[pool["key_idx"]].astype(np.int32)
+    out = []
+    for rate in rates:
+        row = {"offered_verif_per_s": rate}
+        for name, kw, drain in (("continuous", dict(slots=3, max_step_items=8192), max(2.0, seconds)),
+                                ("plain_per_request", dict(slots=1, max_step_items=512, max_requests_per_step=1),
+                                 2.0)):
+            log("serving %s at %d verif/s for %.0f s" % (name, rate, seconds))
+            with blsgpu.Server(hbm_budget_bytes=4 << 30, **kw) as srv:
+                r = srv.loadtest(pool["sig"], pool["nhm"], keys, items_per_s=rate, seconds=seconds, drain_s=drain,
+                                 size_median=16, size_sigma=1.0, size_min=1, size_max=512, seed=seed + rate)
+                st = srv.stats()
+            r = {k: (None if isinstance(v, float) and not np.isfinite(v) else v) for k, v in r.items()}
+            r["steps"] = int(st["steps"])
+            r["mean_step_items"] = st["items"] / max(st["steps"], 1)
+            r["max_steps_in_flight"] = int(st["max_inflight"])
+            row[name] = r
+        out.append(row)
+    return out
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -145,7 +333,7 @@ def main():
     t_gen = time.perf_counter()
     sk, pk, key_idx, cat, sig, nhm = make_workload(wl, n, args.seed)
     lib.blsgpu_reset_keys()
-    handles = np.array([blsgpu.register_key_affine(pk[i]) for i in range(n)], np.int32)
+    handles = blsgpu.register_keys_affine(pk)
     t_gen = time.perf_counter() - t_gen
 
     dev = torch.device("cuda:0")
@@ -253,6 +441,45 @@ def main():
         "setup_s": round(t_gen, 2),
         "engine": {"mode": mode, "lanes_miller": gml, "lanes_final_exp": gfe},
     }
+    forms = set(f for f in args.forms.split(",") if f)
+    t_forms = time.perf_counter()
+    if "batched" in forms or "at_scale" in forms or "serving" in forms:
+        blsgpu.set_batched_engine(12, 16, 0, 0, True)
+    if "batched" in forms:
+        log("batched form: generating the 256-batch workload")
+        w3 = batched_workload(wl, args.seed + 3)
+        lib.blsgpu_reset_keys()
+        h3 = blsgpu.register_keys_affine(w3["pk"])
+        out["batched"] = measure_batched(torch, lib, blsgpu, w3, h3, args.batched_steps, 2, not args.no_tune)
+        out["batched"]["workload"] = ("256 batches, geometric sizes mean 128 on [1,2048], 13 signers, 1024 messages "
+                                      "each signed by 1..13 signers (%d signatures), 1/128 invalid"
+                                      % w3["signatures"])
+        log("batched: %s" % {k: out["batched"][k] for k in ("batched_verif_per_s", "plain_verif_per_s", "speedup")})
+    w4 = None
+    if "at_scale" in forms or "serving" in forms:
+        log("at-scale workload: 65536 items over 40 keys")
+        w4 = at_scale_workload(wl, args.seed + 4)
+        lib.blsgpu_reset_keys()
+        h4 = blsgpu.register_keys_affine(w4["pk"])
+    if "at_scale" in forms:
+        out["at_scale"] = measure_batched(torch, lib, blsgpu, w4, h4, max(2, args.batched_steps // 2), 1,
+                                          not args.no_tune)
+        out["at_scale"]["workload"] = ("65536 verifications, 40 signer keys, 4096 messages, lognormal batch sizes "
+                                       "(median 64, sigma 1.2, [1,8192]), 1/128 invalid")
+        log("at_scale: %s" % {k: out["at_scale"][k] for k in ("batched_verif_per_s", "plain_verif_per_s", "speedup")})
+    if "serving" in forms:
+        pool = serving_pool(wl, args.seed + 5, w4["sk"])
+        rates = [int(r) for r in args.serve_rates.split(",") if r]
+        out["serving"] = {
+            "workload": "Poisson arrivals, request sizes lognormal (median 16, sigma 1.0, [1,512]), 40 keys, fresh "
+                        "messages (pool of %d items, 1/256 invalid), 4 GB HBM budget, %.0f s per run"
+                        % (len(pool["key_idx"]), args.serve_seconds),
+            "latency": "from each request's scheduled arrival to the host seeing its results; null = not finished "
+                       "before the drain deadline",
+            "runs": measure_serving(blsgpu, pool, h4, rates, args.serve_seconds, args.seed),
+        }
+    if forms - {"plain"}:
+        out["forms_s"] = round(time.perf_counter() - t_forms, 1)
     if not args.no_cpu:
         log("cpu baseline (budget %.0f s)" % args.cpu_seconds)
         out["cpu_baseline"] = cpu_baseline(sig, nhm, pk, key_idx, res, args.cpu_seconds)

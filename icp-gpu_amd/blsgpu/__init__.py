@@ -40,6 +40,21 @@ EXTENSION_SYMBOLS = (
     "blsgpu_verify_compressed_device",
     "blsgpu_set_profiling",
     "blsgpu_phase_times",
+    "blsgpu_register_keys_affine",
+    "blsgpu_verify_batches",
+    "blsgpu_verify_batches_device",
+    "blsgpu_set_batched_engine",
+    "blsgpu_get_batched_engine",
+    "blsgpu_batched_stats",
+    "blsgpu_autotune_device",
+    "blsgpu_server_create",
+    "blsgpu_server_step_capacity",
+    "blsgpu_server_submit",
+    "blsgpu_server_poll",
+    "blsgpu_server_wait",
+    "blsgpu_server_stats",
+    "blsgpu_server_destroy",
+    "blsgpu_server_loadtest",
 )
 
 VALID, INVALID = 1, 0
@@ -65,6 +80,7 @@ def load(path=None):
         if not hasattr(lib, name):
             raise RuntimeError("libgpu_msm.so lacks symbol %s" % name)
     vp, i32, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+    i64, dbl = ctypes.c_int64, ctypes.c_double
     sig = {
         "gpu_crypto_init": ([], i32),
         "gpu_batch_bls_verify": ([vp, vp, vp, vp, vp, i32], i32),
@@ -87,6 +103,21 @@ def load(path=None):
         "blsgpu_verify_compressed_device": ([vp, vp, vp, vp, vp, i32, vp], i32),
         "blsgpu_set_profiling": ([i32], i32),
         "blsgpu_phase_times": ([vp, vp], i32),
+        "blsgpu_register_keys_affine": ([vp, i32, vp], i32),
+        "blsgpu_verify_batches": ([vp, vp, vp, vp, i32, vp, vp], i32),
+        "blsgpu_verify_batches_device": ([vp, vp, vp, vp, i32, vp, vp, vp], i32),
+        "blsgpu_set_batched_engine": ([i32, i32, i32, i32, i32], i32),
+        "blsgpu_get_batched_engine": ([i32, vp], i32),
+        "blsgpu_batched_stats": ([vp], i32),
+        "blsgpu_autotune_device": ([vp, vp, vp, i32, vp, vp, vp], i32),
+        "blsgpu_server_create": ([i32, i32, i64, i32, i32], vp),
+        "blsgpu_server_step_capacity": ([vp], i32),
+        "blsgpu_server_submit": ([vp, vp, vp, vp, i32], i64),
+        "blsgpu_server_poll": ([vp, i64, vp, vp, vp], i32),
+        "blsgpu_server_wait": ([vp, i64, vp, vp, vp, dbl], i32),
+        "blsgpu_server_stats": ([vp, vp], i32),
+        "blsgpu_server_destroy": ([vp], None),
+        "blsgpu_server_loadtest": ([vp, vp, vp, vp, i32, dbl, dbl, dbl, dbl, dbl, i32, i32, u64, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -175,3 +206,123 @@ def get_engine():
     m, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     load().blsgpu_get_engine(ctypes.byref(m), ctypes.byref(a), ctypes.byref(b))
     return m.value, a.value, b.value
+
+
+def register_keys_affine(pks):
+    """Registers n keys (n, 24) u64 in one launch; returns their handles."""
+    pks = np.ascontiguousarray(pks, dtype=np.uint64).reshape(-1, 24)
+    h = np.zeros(pks.shape[0], dtype=np.int32)
+    if load().blsgpu_register_keys_affine(_p(pks), pks.shape[0], _p(h)) < 0:
+        raise RuntimeError("register_keys_affine: " + last_error())
+    return h
+
+
+def verify_batches(batches, want_gt=False):
+    """Batched form, host arrays.  batches: list of (sigs48 (n,48) u8, neg_hms (n,12) u64,
+    key_idx (n,) i32).  Returns a list of per-batch results (and GT arrays)."""
+    nb = len(batches)
+    keep = []
+    sp, hp, kp, rp, gp = [(ctypes.c_void_p * nb)() for _ in range(5)]
+    sizes = np.zeros(nb, dtype=np.int32)
+    outs, gts = [], []
+    for b, (sig, nhm, kidx) in enumerate(batches):
+        sig = np.ascontiguousarray(sig, dtype=np.uint8).reshape(-1, 48)
+        nhm = np.ascontiguousarray(nhm, dtype=np.uint64).reshape(-1, 12)
+        kidx = np.ascontiguousarray(kidx, dtype=np.int32)
+        n = sig.shape[0]
+        sizes[b] = n
+        r = np.zeros(n, dtype=np.int32)
+        g = np.zeros((n, 72), dtype=np.uint64) if want_gt else None
+        keep += [sig, nhm, kidx]
+        sp[b], hp[b], kp[b], rp[b] = sig.ctypes.data, nhm.ctypes.data, kidx.ctypes.data, r.ctypes.data
+        if want_gt:
+            gp[b] = g.ctypes.data
+        outs.append(r)
+        gts.append(g)
+    _check(load().blsgpu_verify_batches(ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(hp, ctypes.c_void_p),
+                                        ctypes.cast(kp, ctypes.c_void_p), _p(sizes), nb,
+                                        ctypes.cast(rp, ctypes.c_void_p),
+                                        ctypes.cast(gp, ctypes.c_void_p) if want_gt else None),
+           "blsgpu_verify_batches")
+    return (outs, gts) if want_gt else outs
+
+
+def set_batched_engine(g_miller=12, g_final_exp=16, wpc_miller=0, wpc_final_exp=0, use_graph=True):
+    _check(load().blsgpu_set_batched_engine(g_miller, g_final_exp, wpc_miller, wpc_final_exp, int(use_graph)),
+           "blsgpu_set_batched_engine")
+
+
+def get_batched_engine(m=1):
+    cfg = np.zeros(5, dtype=np.int32)
+    load().blsgpu_get_batched_engine(m, _p(cfg))
+    return tuple(int(v) for v in cfg)
+
+
+def batched_stats():
+    out = np.zeros(2, dtype=np.int32)
+    load().blsgpu_batched_stats(_p(out))
+    return int(out[0]), int(out[1])
+
+
+class Server:
+    """Serving loop (include/blsgpu_serve.h)."""
+
+    def __init__(self, slots=3, max_step_items=8192, hbm_budget_bytes=4 << 30, max_requests_per_step=0,
+                 want_gt=False):
+        lib = load()
+        self.want_gt = want_gt
+        self.h = lib.blsgpu_server_create(slots, max_step_items, hbm_budget_bytes, max_requests_per_step,
+                                          int(want_gt))
+        if not self.h:
+            raise RuntimeError("blsgpu_server_create: " + last_error())
+        self.capacity = lib.blsgpu_server_step_capacity(self.h)
+
+    def submit(self, sigs48, neg_hms, key_idx):
+        sigs48 = np.ascontiguousarray(sigs48, dtype=np.uint8).reshape(-1, 48)
+        neg_hms = np.ascontiguousarray(neg_hms, dtype=np.uint64).reshape(-1, 12)
+        key_idx = np.ascontiguousarray(key_idx, dtype=np.int32)
+        rid = load().blsgpu_server_submit(self.h, _p(sigs48), _p(neg_hms), _p(key_idx), sigs48.shape[0])
+        if rid < 0:
+            raise RuntimeError("blsgpu_server_submit refused the request")
+        return rid, sigs48.shape[0]
+
+    def wait(self, req, timeout_ms=-1.0):
+        rid, n = req
+        res = np.zeros(n, dtype=np.int32)
+        gt = np.zeros((n, 72), dtype=np.uint64) if self.want_gt else None
+        lat = ctypes.c_double()
+        rc = load().blsgpu_server_wait(self.h, rid, _p(res), _p(gt), ctypes.byref(lat), timeout_ms)
+        if rc != 1:
+            raise RuntimeError("request %d: %s" % (rid, "time-out" if rc == 0 else "failed"))
+        return res, gt, lat.value
+
+    def stats(self):
+        out = np.zeros(8, dtype=np.float64)
+        load().blsgpu_server_stats(self.h, _p(out))
+        keys = ("steps", "items", "completed", "max_step", "max_inflight", "capacity", "slots", "failed")
+        return dict(zip(keys, out.tolist()))
+
+    def loadtest(self, pool_sigs48, pool_neg_hms, pool_key_idx, items_per_s, seconds, drain_s=2.0,
+                 size_median=16.0, size_sigma=1.0, size_min=1, size_max=512, seed=7):
+        pool_sigs48 = np.ascontiguousarray(pool_sigs48, dtype=np.uint8).reshape(-1, 48)
+        pool_neg_hms = np.ascontiguousarray(pool_neg_hms, dtype=np.uint64).reshape(-1, 12)
+        pool_key_idx = np.ascontiguousarray(pool_key_idx, dtype=np.int32)
+        out = np.zeros(11, dtype=np.float64)
+        _check(load().blsgpu_server_loadtest(self.h, _p(pool_sigs48), _p(pool_neg_hms), _p(pool_key_idx),
+                                             pool_sigs48.shape[0], items_per_s, seconds, drain_s, size_median,
+                                             size_sigma, size_min, size_max, ctypes.c_uint64(seed), _p(out)),
+               "blsgpu_server_loadtest")
+        keys = ("requests", "items", "requests_done", "items_done", "span_s", "sustained_items_per_s",
+                "p50_ms", "p99_ms", "p999_ms", "max_ms", "offered_items_per_s")
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if self.h:
+            load().blsgpu_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

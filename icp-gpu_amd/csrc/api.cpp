@@ -17,50 +17,16 @@
 #include <vector>
 
 #include "../../include/gpu_msm.h"
+#include "context.hpp"
 #include "kernels.hpp"
 #include "verify_engine.hpp"
 
-namespace {
+namespace blsctx {
 
 thread_local std::string g_err;
+Ctx g;
 
-#define HIPCHK(x)                                                                   \
-  do {                                                                              \
-    hipError_t e_ = (x);                                                            \
-    if (e_ != hipSuccess) {                                                         \
-      g_err = std::string(#x) + ": " + hipGetErrorString(e_);                       \
-      return -1;                                                                    \
-    }                                                                               \
-  } while (0)
-#define CHK(x)                                                                      \
-  do {                                                                              \
-    if ((x) != 0) {                                                                 \
-      if (g_err.empty()) g_err = std::string(#x) + " failed";                       \
-      return -1;                                                                    \
-    }                                                                               \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  int ensure(size_t n) {
-    if (n <= cap) return 0;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    size_t want = n + n / 4 + 4096;
-    if (hipMalloc(&p, want) != hipSuccess) {
-      g_err = "hipMalloc failed";
-      p = nullptr;
-      return -1;
-    }
-    cap = want;
-    return 0;
-  }
-  template <class T>
-  T* as() const { return reinterpret_cast<T*>(p); }
-};
-
+namespace {
 // Standard G2 generator in ic_bls12_381's Montgomery form (R = 2^384).
 const uint64_t G2_GEN_REF[24] = {
     0xf5f28fa202940a10ULL, 0xb3f5fb2687b4961aULL, 0xa1a893b53e2ae580ULL, 0x9894999d1a3caee9ULL,
@@ -70,31 +36,8 @@ const uint64_t G2_GEN_REF[24] = {
     0x07d3a975f0ef25a2ULL, 0x0083fd8e7e80dae5ULL, 0xadc0fc92df64b05dULL, 0x18aa270a2b1461dcULL,
     0x86adac6a3be4eba0ULL, 0x79495c4ec93da33aULL, 0xe7175850a43ccaedULL, 0x0b2bc2a163de1bf2ULL};
 
-struct Ctx {
-  std::mutex mu;
-  int state = 0;  // 0 = not initialized, 1 = ready, -1 = no device
-  int device = 0;
-  hipStream_t s = nullptr;
-  // per-call staging
-  DevBuf sig_ref, nhm_ref, g2_ref, pk_ref, comp, keyidx;
-  DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, sub_status, fbuf, res, gt;
-  hipStream_t s2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  // resident
-  DevBuf gen_lines;  // G2 generator lines (internal), computed at init
-  DevBuf keys;       // registered key tables (internal, key-major)
-  int nkeys = 0;
-  // device-form calls: the stream the workspace was last used on, and timing events
-  hipStream_t last = nullptr;
-  bool profiling = false;
-  std::vector<hipEvent_t> ev_free;
-  std::vector<std::vector<hipEvent_t>> ev_pending;
-  double phase_ms[4] = {0, 0, 0, 0};
-  int phase_calls = 0;
-  // engine configuration (blsgpu_set_engine)
-  int mode = 1, g_ml = 16, g_fe = 16;
-};
-Ctx g;
+
+}  // namespace
 
 // Workspace buffers are shared by all calls; before they are reused (or grown) from a
 // different stream, wait for the stream that last used them.
@@ -106,7 +49,10 @@ int claim_workspace_locked(hipStream_t s) {
 
 int init_locked() {
   if (g.state == 1) return 0;
-  if (g.state == -1) return -1;
+  if (g.state == -1) {
+    g_err = "no HIP device";
+    return -1;
+  }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
     g.state = -1;
@@ -181,24 +127,15 @@ int grow_keys_locked(int add) {
   if (g.nkeys) HIPCHK(hipMemcpyAsync(np, g.keys.p, (size_t)g.nkeys * bls::KEY_TABLE_WORDS * 4,
                                       hipMemcpyDeviceToDevice, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
-  if (g.keys.p) (void)hipFree(g.keys.p);
+  if (g.keys.p) g.retired_keys.push_back(g.keys.p);
   g.keys.p = np;
   g.keys.cap = want;
   return 0;
 }
 
-}  // namespace
+}  // namespace blsctx
 
-namespace bls {
-// accessors used by the device-pointer entry points (device_api.cpp)
-hipStream_t lib_stream() { return g.s; }
-int lib_init_locked() { return init_locked(); }
-std::mutex& lib_mutex() { return g.mu; }
-const int32_t* lib_gen_lines() { return g.gen_lines.as<int32_t>(); }
-const int32_t* lib_key_tables() { return g.keys.as<int32_t>(); }
-int lib_num_keys() { return g.nkeys; }
-void lib_set_error(const char* m) { g_err = m; }
-}  // namespace bls
+using namespace blsctx;
 
 extern "C" {
 
@@ -297,6 +234,28 @@ int blsgpu_register_key_affine(const uint64_t pk[24]) {
                              1, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
   return g.nkeys++;
+}
+
+int blsgpu_register_keys_affine(const uint64_t* pks, int n, int32_t* handles) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (!pks || n <= 0) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  CHK(init_locked());
+  CHK(grow_keys_locked(n));
+  CHK(claim_workspace_locked(g.s));
+  CHK(g.pk_ref.ensure((size_t)n * 24 * 8));
+  HIPCHK(hipMemcpyAsync(g.pk_ref.p, pks, (size_t)n * 24 * 8, hipMemcpyHostToDevice, g.s));
+  CHK(bls::launch_g2_prepare(g.pk_ref.as<uint64_t>(), g.keys.as<int32_t>() + (size_t)g.nkeys * bls::KEY_TABLE_WORDS,
+                             n, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  const int first = g.nkeys;
+  g.nkeys += n;
+  if (handles)
+    for (int i = 0; i < n; i++) handles[i] = first + i;
+  return first;
 }
 
 int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]) {

@@ -25,11 +25,18 @@ int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* res
 
 // lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13,
 // 16, 32; final exponentiation: 12, 16, 18, 32).  Return -1 for an unsupported G or a launch error.
+// queue == nullptr: one workgroup per tile of 64/G items.  Otherwise a persistent grid
+// (at most wg_per_cu workgroups per CU, 0 = the occupancy limit) pops tiles from the
+// device counter *queue, which must be zero at launch.
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                       const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                      hipStream_t s);
+                      int* queue, int wg_per_cu, hipStream_t s);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, hipStream_t s);
+                         uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s);
+// resident workgroups per CU of a grouped kernel (which: 0 Miller, 1 final exp), -1 if G is unsupported
+int occupancy_grp(int which, int G);
+// p[0..n) = 0 by a kernel (safe inside captured graphs)
+int launch_zero_words(int* p, int n, hipStream_t s);
 
 // G1 decoding in two parts (kernels_g1.hip): decode (flags, range, square root, sign)
 // and the subgroup test, which may run concurrently with the Miller loop.

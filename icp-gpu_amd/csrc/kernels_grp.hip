@@ -162,24 +162,21 @@ constexpr size_t miller_lds_bytes() {
 }
 
 template <int G>
-__global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
-                                                   const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
-                                                   const int32_t* __restrict__ key_idx, int nkeys,
-                                                   const i32* __restrict__ pk_soa, i32* __restrict__ status,
-                                                   i32* __restrict__ fout, int M) {
+__device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ sig, const i32* __restrict__ nhm,
+                                            const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
+                                            const int32_t* __restrict__ key_idx, int nkeys,
+                                            const i32* __restrict__ pk_soa, i32* __restrict__ status,
+                                            i32* __restrict__ fout, int M) {
   constexpr int IPW = 64 / G;
   const int lid = threadIdx.x / G, sub = threadIdx.x % G;
   const bool in_wg = lid < IPW;
-  const int item = blockIdx.x * IPW + lid;
+  const int item = tile * IPW + lid;
   const bool valid = in_wg && item < M;
   const int it = valid ? item : 0;
   const int base = (in_wg ? lid : 0) * ML_SLOTS * SLOT;
   const int F = base, L1 = base + 12 * SLOT, L2 = base + 18 * SLOT, P = base + 24 * SLOT;
   const int S1 = base + 28 * SLOT, S2 = base + 34 * SLOT, T = base + 40 * SLOT;
   const int K = IPW * ML_SLOTS * SLOT;
-  load_consts(K);
-  load_schedule(K + sched::NUM_CONSTS * SLOT);
-
   bool skip_all = !valid, skip1 = false;
   int kidx = 0;
   if (valid) {
@@ -246,6 +243,38 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
       fout[(size_t)w * M + item] = g_lds[F + e * SLOT + l];
     }
   }
+  __syncthreads();  // the next tile reuses this tile's LDS
+}
+
+// Next tile of a workgroup: blockIdx.x alone (one tile per workgroup), or, with a work
+// queue, tiles popped from a device counter until every tile is taken.  Every wave of
+// the workgroup leaves at the same tile index, so the barriers inside stay uniform.
+__device__ __forceinline__ int next_tile(int* queue, int round) {
+  __shared__ int s_tile;
+  if (!queue) return round == 0 ? (int)blockIdx.x : INT_MAX;
+  if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
+  __syncthreads();
+  const int t = s_tile;
+  __syncthreads();
+  return t;
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
+                                                   const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
+                                                   const int32_t* __restrict__ key_idx, int nkeys,
+                                                   const i32* __restrict__ pk_soa, i32* __restrict__ status,
+                                                   i32* __restrict__ fout, int M, int* queue) {
+  constexpr int IPW = 64 / G;
+  const int ntiles = (M + IPW - 1) / IPW;
+  const int K = IPW * ML_SLOTS * SLOT;
+  load_consts(K);
+  load_schedule(K + sched::NUM_CONSTS * SLOT);
+  for (int round = 0;; round++) {
+    const int tile = next_tile(queue, round);
+    if (tile >= ntiles) break;
+    miller_tile<G>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, pk_soa, status, fout, M);
+  }
 }
 
 // ---------------- final exponentiation ----------------
@@ -269,13 +298,13 @@ __device__ __forceinline__ void cyc_exp_grp(int sub, bool act, int src, int dst,
 }
 
 template <int G>
-__global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
-                                                      const i32* __restrict__ sub_status, int32_t* __restrict__ result,
-                                                      u64* __restrict__ gt_out, int M) {
+__device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__ fin, const i32* __restrict__ status,
+                                               const i32* __restrict__ sub_status, int32_t* __restrict__ result,
+                                               u64* __restrict__ gt_out, int M) {
   constexpr int IPW = 64 / G;
   const int lid = threadIdx.x / G, sub = threadIdx.x % G;
   const bool in_wg = lid < IPW;
-  const int item = blockIdx.x * IPW + lid;
+  const int item = tile * IPW + lid;
   const bool valid = in_wg && item < M;
   const int it = valid ? item : 0;
   const int base = (in_wg ? lid : 0) * FE_SLOTS * SLOT;
@@ -283,8 +312,6 @@ __global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fi
   const int R4 = base + 48 * SLOT, R5 = base + 60 * SLOT, T = base + 72 * SLOT;
   const int K = IPW * FE_SLOTS * SLOT;
   const int FLAGS = K + sched::NUM_CONSTS * SLOT + SCHED_WORDS + (in_wg ? lid : 0) * 12;
-  load_consts(K);
-  load_schedule(K + sched::NUM_CONSTS * SLOT);
   int32_t st = DEC_OK;
   if (valid && status) st = status[it];
   if (valid && sub_status && st == DEC_OK) st = sub_status[it];
@@ -380,48 +407,126 @@ __global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fi
     for (int k = 0; k < 12; k++) all &= g_lds[FLAGS + k];
     result[it] = act ? all : st;
   }
+  __syncthreads();  // the next tile reuses this tile's LDS
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
+                                                      const i32* __restrict__ sub_status, int32_t* __restrict__ result,
+                                                      u64* __restrict__ gt_out, int M, int* queue) {
+  constexpr int IPW = 64 / G;
+  const int ntiles = (M + IPW - 1) / IPW;
+  const int K = IPW * FE_SLOTS * SLOT;
+  load_consts(K);
+  load_schedule(K + sched::NUM_CONSTS * SLOT);
+  for (int round = 0;; round++) {
+    const int tile = next_tile(queue, round);
+    if (tile >= ntiles) break;
+    final_exp_tile<G>(tile, fin, status, sub_status, result, gt_out, M);
+  }
 }
 
 // ---------------- launch wrappers ----------------
+// Resident workgroups per CU of each kernel instantiation, queried once (callers hold
+// the library lock) so that no occupancy query happens during graph capture.
+static int num_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+  }
+  return cus;
+}
+template <int G>
+static int ml_occ() {
+  static int occ = 0;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G>, 64, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  return occ;
+}
+template <int G>
+static int fe_occ() {
+  static int occ = 0;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G>, 64, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  return occ;
+}
+// Grid of a queue-fed launch: at most wg_per_cu workgroups per CU (0 = as many as
+// fit), never more than the occupancy limit, never more than there are tiles.
+static int queue_grid(int occ, int ntiles, int wg_per_cu) {
+  const int per = wg_per_cu > 0 && wg_per_cu < occ ? wg_per_cu : occ;
+  const long g = (long)num_cus() * per;
+  return (int)(g < ntiles ? g : ntiles);
+}
+
 template <int G>
 static int launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                     hipStream_t s) {
+                     int* queue, int wg_per_cu, hipStream_t s) {
   constexpr int IPW = 64 / G;
-  hipLaunchKernelGGL(k_miller_grp<G>, dim3((M + IPW - 1) / IPW), dim3(64), miller_lds_bytes<G>(), s, sig, nhm, gen,
-                     keys, key_idx, nkeys, pk_soa, status, fout, M);
+  const int ntiles = (M + IPW - 1) / IPW;
+  const int grid = queue ? queue_grid(ml_occ<G>(), ntiles, wg_per_cu) : ntiles;
+  hipLaunchKernelGGL(k_miller_grp<G>, dim3(grid), dim3(64), miller_lds_bytes<G>(), s, sig, nhm, gen, keys, key_idx,
+                     nkeys, pk_soa, status, fout, M, queue);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int G>
 static int launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                     uint64_t* gt, int M, hipStream_t s) {
+                     uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
   constexpr int IPW = 64 / G;
-  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3((M + IPW - 1) / IPW), dim3(64), final_exp_lds_bytes<G>(), s, fin, status,
-                     sub_status, result, gt, M);
+  const int ntiles = (M + IPW - 1) / IPW;
+  const int grid = queue ? queue_grid(fe_occ<G>(), ntiles, wg_per_cu) : ntiles;
+  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3(grid), dim3(64), final_exp_lds_bytes<G>(), s, fin, status, sub_status,
+                     result, gt, M, queue);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                       const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                      hipStream_t s) {
+                      int* queue, int wg_per_cu, hipStream_t s) {
   switch (G) {
-    case 8: return launch_ml<8>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
-    case 12: return launch_ml<12>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
-    case 13: return launch_ml<13>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
-    case 16: return launch_ml<16>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
-    case 32: return launch_ml<32>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, s);
+#define ML_CASE(g) \
+  case g: return launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, s);
+    ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(32)
+#undef ML_CASE
     default: return -1;
   }
 }
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, hipStream_t s) {
+                         uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
   switch (G) {
-    case 12: return launch_fe<12>(fin, status, sub_status, result, gt, M, s);
-    case 16: return launch_fe<16>(fin, status, sub_status, result, gt, M, s);
-    case 18: return launch_fe<18>(fin, status, sub_status, result, gt, M, s);
-    case 32: return launch_fe<32>(fin, status, sub_status, result, gt, M, s);
+#define FE_CASE(g) case g: return launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, s);
+    FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(32)
+#undef FE_CASE
     default: return -1;
   }
+}
+
+int occupancy_grp(int which, int G) {
+  if (which == 0) {
+    switch (G) {
+      case 8: return ml_occ<8>();
+      case 12: return ml_occ<12>();
+      case 13: return ml_occ<13>();
+      case 16: return ml_occ<16>();
+      case 32: return ml_occ<32>();
+      default: return -1;
+    }
+  }
+  switch (G) {
+    case 12: return fe_occ<12>();
+    case 16: return fe_occ<16>();
+    case 18: return fe_occ<18>();
+    case 32: return fe_occ<32>();
+    default: return -1;
+  }
+}
+
+__global__ void k_zero_words(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+int launch_zero_words(int* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_words, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace bls

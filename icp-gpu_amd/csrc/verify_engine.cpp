@@ -10,14 +10,16 @@ static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, 
   if (a.mode == 0)
     return launch_miller(a.sig_pts, a.nhm_pts, gen, key_idx ? keys : pk_soa, key_idx, a.nkeys, status, a.fbuf, a.M,
                          a.block, s);
-  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_soa, status, a.fbuf, a.M, s);
+  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_soa, status, a.fbuf, a.M,
+                           a.queue && a.wpc_ml >= 0 ? a.queue : nullptr, a.wpc_ml, s);
 }
 static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* sub_status, hipStream_t s) {
   if (a.mode == 0) {
     if (sub_status && launch_merge_status(const_cast<int32_t*>(status), sub_status, a.M, s)) return -1;
     return launch_final_exp(a.fbuf, status, a.results, a.gt, a.M, a.block, s);
   }
-  return launch_final_exp_grp(a.g_fe, a.fbuf, status, sub_status, a.results, a.gt, a.M, s);
+  return launch_final_exp_grp(a.g_fe, a.fbuf, status, sub_status, a.results, a.gt, a.M,
+                              a.queue && a.wpc_fe >= 0 ? a.queue + 1 : nullptr, a.wpc_fe, s);
 }
 
 int engine_legacy(const EngineArgs& a, hipStream_t s) {
@@ -35,7 +37,9 @@ static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
 
 // decode -> fork: [subgroup test on s2] || [Miller loop on s] -> join -> final exp
 int engine_compressed(const EngineArgs& a, hipStream_t s) {
-  if (mark(a, 0, s) || launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
+  if (mark(a, 0, s)) return -1;
+  if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 2, s)) return -1;
+  if (launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
   if (mark(a, 1, s) || launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
   const bool fork = a.s2 && a.fork && a.join;
   if (fork) {
@@ -49,6 +53,20 @@ int engine_compressed(const EngineArgs& a, hipStream_t s) {
   if (fork && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
   if (mark(a, 3, s) || final_exp(a, a.status, a.sub_status, s)) return -1;
   return mark(a, 4, s);
+}
+
+int engine_prepare(const EngineArgs& a, hipStream_t s) {
+  if (launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
+  if (launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
+  return launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, s);
+}
+int engine_miller(const EngineArgs& a, hipStream_t s) {
+  if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 1, s)) return -1;
+  return miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s);
+}
+int engine_final_exp(const EngineArgs& a, hipStream_t s) {
+  if (a.queue && a.mode == 1 && launch_zero_words(a.queue + 1, 1, s)) return -1;
+  return final_exp(a, a.status, a.sub_status, s);
 }
 
 int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s) {

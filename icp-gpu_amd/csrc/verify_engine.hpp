@@ -34,6 +34,12 @@ struct EngineArgs {
   int mode = 1;
   int g_ml = 16;
   int g_fe = 16;
+  // persistent form: two device counters (Miller, final exp) that the sequence zeroes
+  // itself, and the workgroups per CU of each kernel (0 = occupancy limit, -1 = that
+  // kernel is launched with one workgroup per tile and no queue)
+  int* queue = nullptr;
+  int wpc_ml = 0;
+  int wpc_fe = 0;
   // optional timing: events recorded before each phase and at the end (5 events)
   hipEvent_t* ev = nullptr;
   // second stream for the subgroup test, and the fork/join events (fused path)
@@ -45,5 +51,10 @@ int engine_legacy(const EngineArgs& a, hipStream_t s);
 int engine_compressed(const EngineArgs& a, hipStream_t s);
 // decode (and optionally subgroup-check) a.comp into a.sig_pts / a.status
 int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s);
+// the two halves of engine_compressed, for the autotuner: decode, -H(m) conversion and
+// the subgroup test (all on s); then the Miller loop alone and the final exponentiation alone
+int engine_prepare(const EngineArgs& a, hipStream_t s);
+int engine_miller(const EngineArgs& a, hipStream_t s);
+int engine_final_exp(const EngineArgs& a, hipStream_t s);
 
 }  // namespace bls

@@ -269,6 +269,11 @@ int blsgpu_wl_public_keys(const uint8_t* sk32, int n, uint64_t* pk_ref) {
 
 int blsgpu_wl_items(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
                     const int8_t* category, uint8_t* sig48, uint64_t* nhm_ref) {
+  return blsgpu_wl_items_msg(seed, n, sk32, nkeys, key_idx, nullptr, category, sig48, nhm_ref);
+}
+
+int blsgpu_wl_items_msg(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
+                        const int64_t* msg_idx, const int8_t* category, uint8_t* sig48, uint64_t* nhm_ref) {
   std::lock_guard<std::mutex> lk(wl_mu);
   if (n <= 0 || nkeys <= 0) return -1;
   for (int i = 0; i < n; i++)
@@ -279,7 +284,11 @@ int blsgpu_wl_items(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const 
   std::vector<uint8_t> hs((size_t)n * 32), ho((size_t)n * 32, 0);
   uint8_t msg[32];
   for (int i = 0; i < n; i++) {
-    rand_bytes(seed, 1000000ULL + (u64)i, msg, 32);
+    if (msg_idx && msg_idx[i] < 0) {
+      wl_err = "bad message index";
+      return -1;
+    }
+    rand_bytes(seed, 1000000ULL + (u64)(msg_idx ? msg_idx[i] : i), msg, 32);
     hash_scalar(msg, 32, &hs[(size_t)i * 32]);
     if (category[i] == BLSGPU_WL_WRONG_MSG) {
       rand_bytes(seed, 2000000ULL + (u64)i, msg, 32);

@@ -27,6 +27,10 @@ int blsgpu_wl_public_keys(const uint8_t* sk32, int n, uint64_t* pk_ref);
  * item i signs message i with key key_idx[i] unless its category says otherwise */
 int blsgpu_wl_items(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
                     const int8_t* category, uint8_t* sig48, uint64_t* nhm_ref);
+/* as blsgpu_wl_items, but item i signs message msg_idx[i] (messages shared by several
+ * signers); msg_idx NULL = item index */
+int blsgpu_wl_items_msg(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
+                        const int64_t* msg_idx, const int8_t* category, uint8_t* sig48, uint64_t* nhm_ref);
 /* measured v_mad_u64_u32 issue rate over all CUs (products per second) */
 int blsgpu_probe_mad_rate(double* mads_per_s, double* ms);
 const char* blsgpu_wl_last_error(void);

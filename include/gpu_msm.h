@@ -78,6 +78,9 @@ int blsgpu_batch_verify_gt(const uint64_t* sigs, const uint64_t* neg_hms, const 
  * from_coeffs takes the 68 precomputed lines (2448 u64). */
 int blsgpu_register_key_affine(const uint64_t pk[24]);
 int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]);
+/* n keys from affine points (n x 24 u64) in one launch; handles[i] (nullable) receives
+ * the handle of key i.  Returns the first handle, or -1. */
+int blsgpu_register_keys_affine(const uint64_t* pks, int n, int32_t* handles);
 /* copies a registered key's lines back out (2448 u64, reference form) */
 int blsgpu_key_coeffs(int handle, uint64_t coeffs[2448]);
 int blsgpu_num_keys(void);
@@ -105,11 +108,48 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
                                     const int32_t* d_key_idx, int32_t* d_results, uint64_t* d_gt,
                                     int M, void* stream);
 
+/* Batched form: nbatches verify batches in one submission, with the same per-item
+ * semantics as blsgpu_verify_compressed.  The Miller-loop and final-exponentiation
+ * kernels are persistent (a device-sized grid pops tiles of items from a device work
+ * queue), and the whole kernel sequence is captured once as a hipGraph per input shape
+ * and replayed.  No counterpart in the reference (its entry point launches one kernel
+ * per batch, gpu-crypto/cuda/api.cu:128-160).
+ *   _device: inputs and outputs are device pointers; batch b is items
+ *     [batch_off[b], batch_off[b+1]) of the concatenated arrays (batch_off: host array
+ *     of nbatches + 1 non-decreasing entries, batch_off[0] = 0); the work is ordered on
+ *     `stream` (NULL = the library's stream) and the call returns without waiting.
+ *   host form: batch b has sizes[b] items at sigs48[b], neg_hms[b], key_idx[b] and its
+ *     results go to results[b] (gt_out nullable, else gt_out[b] nullable per batch);
+ *     synchronous. */
+int blsgpu_verify_batches_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx,
+                                 const int32_t* batch_off, int nbatches, int32_t* d_results, uint64_t* d_gt,
+                                 void* stream);
+int blsgpu_verify_batches(const uint8_t* const* sigs48, const uint64_t* const* neg_hms,
+                          const int32_t* const* key_idx, const int32_t* sizes, int nbatches,
+                          int32_t* const* results, uint64_t* const* gt_out);
+/* Batched-form configuration: lanes per item of the Miller loop (8, 12, 13, 16, 32) and of
+ * the final exponentiation (12, 16, 18, 32), workgroups per CU of each (0 = occupancy
+ * limit, -1 = one workgroup per tile, no queue), and graph capture on/off.  Clears the
+ * autotuned table. */
+int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int wpc_final_exp, int use_graph);
+/* configuration used for a job of M items: {g_miller, g_final_exp, wpc_miller, wpc_final_exp, use_graph} */
+int blsgpu_get_batched_engine(int M, int cfg[5]);
+/* out[0] graphs instantiated, out[1] graph launches */
+int blsgpu_batched_stats(int out[2]);
+/* Autotuner: times every lanes-per-item and workgroups-per-CU candidate of the Miller
+ * loop, then of the final exponentiation, on the given device-resident job (M items),
+ * and records the fastest pair for M's size class (ceil(log2 M)); later batched calls
+ * and serving steps use the entry of the nearest class.  cfg_out (nullable) =
+ * {g_miller, g_final_exp, wpc_miller, wpc_final_exp}; ms_out (nullable) = the two best
+ * kernel times in ms.  Synchronous. */
+int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx, int M,
+                           void* stream, int cfg_out[4], double ms_out[2]);
+
 /* Engine selection: mode 0 = one thread per verification (thread-per-item kernels),
  * mode 1 = a group of G lanes of one wavefront per verification (g_miller in
  * {8, 12, 13, 16, 32} for the Miller loop, g_final_exp in {12, 16, 18, 32} for the final
  * exponentiation).  The
- * default is mode 1 with 16/16; the autotuner in the Python binding picks per batch size. */
+ * default is mode 1 with 12/16. */
 int blsgpu_set_engine(int mode, int g_miller, int g_final_exp);
 int blsgpu_get_engine(int* mode, int* g_miller, int* g_final_exp);
 

@@ -185,17 +185,18 @@ def sign(sk, msg):
     return s, h
 
 
-def make_batch(seed, n_items, sks, key_idx, invalid=None):
+def make_batch(seed, n_items, sks, key_idx, invalid=None, msg_idx=None):
     """Signatures over fresh 32-byte messages.  invalid: dict index -> category in
     {'wrong_msg', 'no_sqrt', 'non_subgroup'}.  Returns compressed sigs (n,48), uncompressed
-    sigs (n,12) (zeros where the encoding is undecodable), neg_hms (n,12), messages."""
+    sigs (n,12) (zeros where the encoding is undecodable), neg_hms (n,12), messages.
+    msg_idx: message index of each item (default: the item index)."""
     invalid = invalid or {}
     comp = np.zeros((n_items, 48), np.uint8)
     unc = np.zeros((n_items, 12), np.uint64)
     nhm = np.zeros((n_items, 12), np.uint64)
     msgs = []
     for i in range(n_items):
-        msg = rand_bytes(seed, 1_000_000 + i, 32)
+        msg = rand_bytes(seed, 1_000_000 + (i if msg_idx is None else int(msg_idx[i])), 32)
         msgs.append(msg)
         sk = sks[key_idx[i]]
         cat = invalid.get(i)

@@ -64,3 +64,29 @@ def test_invalid_sizes_rejected_before_any_device_work():
 def test_init_reports_no_device():
     # gpu-crypto/cuda/api.cu:118-124: -1 when no device is found
     assert blsgpu.init() == -1
+
+
+def test_batched_and_serving_entry_points_validate_arguments():
+    lib = blsgpu.load()
+    off = (ctypes.c_int32 * 3)(0, 5, 3)  # decreasing offsets
+    assert lib.blsgpu_verify_batches(None, None, None, None, 0, None, None) == -1
+    assert lib.blsgpu_verify_batches_device(None, None, None, ctypes.cast(off, ctypes.c_void_p), 2, None, None,
+                                            None) == -1
+    assert lib.blsgpu_set_batched_engine(11, 16, 0, 0, 1) == -1
+    assert lib.blsgpu_set_batched_engine(12, 17, 0, 0, 1) == -1
+    assert lib.blsgpu_set_batched_engine(12, 16, -2, 0, 1) == -1
+    assert lib.blsgpu_set_batched_engine(13, 18, 1, -1, 0) == 0
+    cfg = (ctypes.c_int * 5)()
+    assert lib.blsgpu_get_batched_engine(100, ctypes.cast(cfg, ctypes.c_void_p)) == 0
+    assert list(cfg) == [13, 18, 1, -1, 0]
+    assert lib.blsgpu_set_batched_engine(12, 16, 0, 0, 1) == 0
+    assert not lib.blsgpu_server_create(0, 16, 1 << 20, 0, 0)
+    assert not lib.blsgpu_server_create(2, 0, 1 << 20, 0, 0)
+    assert lib.blsgpu_server_submit(None, None, None, None, 1) == -1
+    assert lib.blsgpu_server_poll(None, 0, None, None, None) == -1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_server_needs_a_device():
+    assert not blsgpu.load().blsgpu_server_create(2, 64, 1 << 20, 0, 0)
+    assert "device" in blsgpu.last_error()

@@ -75,5 +75,5 @@ def test_bad_key_index_is_reported():
 
 
 def test_restore_default_engine():
-    blsgpu.set_engine(1, 16, 16)
-    assert blsgpu.get_engine() == (1, 16, 16)
+    blsgpu.set_engine(1, 12, 16)
+    assert blsgpu.get_engine() == (1, 12, 16)
