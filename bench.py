@@ -54,8 +54,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--engine", default="1:12:16", help="mode:lanes_miller:lanes_final_exp")
-    ap.add_argument("--sweep", action="store_true", help="time every engine configuration first")
+    ap.add_argument("--engine", default="auto",
+                    help="mode:lanes_miller:lanes_final_exp, or auto (the library's autotuner on this batch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
     ap.add_argument("--forms", default="plain,batched,at_scale,serving",
@@ -97,17 +97,20 @@ def expected_codes(cat):
 
 
 def cpu_baseline(sig, nhm, pk, key_idx, gpu_res, budget_s):
-    """Oracle (CPU restatement of the reference's path) on a bounded sample."""
+    """Oracle (CPU restatement of the reference's path) on a bounded sample: the batch,
+    repeated until the sample holds about budget_s seconds of work on the host's cores
+    (key line tables precomputed outside the timing, as the GPU's are)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
     cores = min(16, os.cpu_count() or 1)
     g2c = O.g2_prepare(O.g2_gen())
-    # one single-thread timing to size the sample
+    keys = np.unique(key_idx)
+    remap = np.zeros(int(keys.max()) + 1, np.int32)
+    remap[keys] = np.arange(len(keys), dtype=np.int32)
+    kc = np.ascontiguousarray(np.array([O.g2_prepare(pk[k]) for k in keys]))
+
     def run(idx, threads):
-        keys = np.unique(key_idx[idx])
-        remap = {k: j for j, k in enumerate(keys)}
-        kc = np.ascontiguousarray(np.array([O.g2_prepare(pk[k]) for k in keys]))
-        ki = np.array([remap[k] for k in key_idx[idx]], np.int32)
+        ki = np.ascontiguousarray(remap[key_idx[idx]])
         s = np.ascontiguousarray(sig[idx])
         h = np.ascontiguousarray(nhm[idx])
         res = np.zeros(len(idx), np.int32)
@@ -115,20 +118,23 @@ def cpu_baseline(sig, nhm, pk, key_idx, gpu_res, budget_s):
         O.lib().orc_verify_compressed_batch(O.ptr(s), O.ptr(h), O.ptr(g2c), O.ptr(kc), O.ptr(ki), O.ptr(res),
                                            len(idx), threads)
         return res, time.perf_counter() - t0
-    _, t1 = run(np.arange(2), 1)
-    per = t1 / 2
-    n_sample = int(min(len(sig), max(cores, budget_s * cores / per)))
-    n_sample -= n_sample % cores if n_sample > cores else 0
-    idx = np.arange(n_sample)
+    _, t1 = run(np.arange(4), 1)
+    per = t1 / 4
+    n = len(sig)
+    want = max(cores, int(budget_s * cores / per))
+    reps = max(1, -(-want // n))
+    idx = np.arange(want) if want < n else np.tile(np.arange(n), reps)
     res, t = run(idx, cores)
     agree = bool((res == gpu_res[idx]).all())
     return {
-        "value": n_sample / t,
+        "value": len(idx) / t,
         "unit": "verifications/s",
         "cores": cores,
         "kind": "port",
-        "sample": "first %d items of the same 4096-item batch (decompress+checks+verify), %.1f s; "
-                  "single-thread %.2f ms/verify; verdicts agree with GPU: %s" % (n_sample, t, per * 1e3, agree),
+        "sample": "%d verifications: the %d-item batch %s (decompress+checks+verify, key lines precomputed), %.1f s "
+                  "on %d threads; single-thread %.2f ms/verify; verdicts agree with GPU: %s"
+                  % (len(idx), n, "repeated %d times" % reps if want >= n else "(first %d items)" % len(idx), t,
+                     cores, per * 1e3, agree),
         "single_thread_ms": per * 1e3,
         "verdicts_agree": agree,
     }
@@ -230,7 +236,7 @@ def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune):
         cfg = np.zeros(4, np.int32)
         ms = np.zeros(2, np.float64)
         t_t = time.perf_counter()
-        if lib.blsgpu_autotune_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(), n, sp,
+        if lib.blsgpu_autotune_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(), n, 0, sp,
                                       cfg.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p)):
             raise RuntimeError("autotune: " + blsgpu.last_error())
         tuned = {"lanes_miller": int(cfg[0]), "lanes_final_exp": int(cfg[1]), "wg_per_cu_miller": int(cfg[2]),
@@ -304,6 +310,22 @@ def measure_serving(blsgpu, pool, handles, rates, seconds, seed):
     return out
 
 
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summaries
+    (a line 'traffic_bytes_per_launch <kernel> <bytes>' in profiles/*.md), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, name)):
+            f = line.strip().strip("`").split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return {"bytes": float(f[2]), "source": "profiles/" + name}
+    return None
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -350,20 +372,18 @@ def main():
         if rc != 0:
             raise RuntimeError(blsgpu.last_error())
 
-    if args.sweep:
-        timings = {}
-        for cfg in [(1, 12, 16), (1, 13, 16), (1, 16, 16), (1, 13, 12), (1, 13, 18), (1, 16, 18)]:
-            blsgpu.set_engine(*cfg)
-            step()
-            torch.cuda.synchronize()
-            t_a = time.perf_counter()
-            for _ in range(3):
-                step()
-            torch.cuda.synchronize()
-            timings["%d:%d:%d" % cfg] = (time.perf_counter() - t_a) * 1e3 / 3
-            log("sweep %s: %.3f ms/step" % ("%d:%d:%d" % cfg, timings["%d:%d:%d" % cfg]))
-        best = min(timings, key=timings.get)
-        args.engine = best
+    tuning = None
+    if args.engine == "auto":
+        # the library's autotuner, plain form: lanes per item of each kernel, timed on this batch
+        cfg = np.zeros(4, np.int32)
+        ms = np.zeros(2, np.float64)
+        t_t = time.perf_counter()
+        if lib.blsgpu_autotune_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(), n, 1, sp,
+                                      cfg.ctypes.data_as(vp), ms.ctypes.data_as(vp)):
+            raise RuntimeError("autotune: " + blsgpu.last_error())
+        tuning = {"miller_ms": float(ms[0]), "final_exp_ms": float(ms[1]),
+                  "tuning_s": round(time.perf_counter() - t_t, 2)}
+        args.engine = "1:%d:%d" % (cfg[0], cfg[1])
     mode, gml, gfe = (int(x) for x in args.engine.split(":"))
     blsgpu.set_engine(mode, gml, gfe)
     log("engine %s" % args.engine)
@@ -414,11 +434,16 @@ def main():
         "unit": "G(32x32-bit products)/s",
         "frac": achieved / mads.value,
         "traffic": None,
+        "traffic_source": None,
         "peak_theoretical": MI355X_CUS * 64 * MI355X_GHZ,
         "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, phase_avg)},
         "whole_job_frac": value * (FP_MULS_MILLER + FP_MULS_FINAL_EXP) * PRODUCTS_PER_FP_MUL / mads.value,
     }
 
+    tr = read_traffic("%s_g%d" % (names[dom], gfe if names[dom] == "final_exp" else gml))
+    if tr:
+        roofline["traffic"] = tr["bytes"]
+        roofline["traffic_source"] = tr["source"]
     out = {
         "metric": "BLS sig verifications/sec on 1 GPU, batched vs plain; p50/p99 latency at load",
         "value": value,
@@ -439,7 +464,7 @@ def main():
         "verdicts_as_expected": ok,
         "roofline": roofline,
         "setup_s": round(t_gen, 2),
-        "engine": {"mode": mode, "lanes_miller": gml, "lanes_final_exp": gfe},
+        "engine": {"mode": mode, "lanes_miller": gml, "lanes_final_exp": gfe, "autotuned": tuning},
     }
     forms = set(f for f in args.forms.split(",") if f)
     t_forms = time.perf_counter()

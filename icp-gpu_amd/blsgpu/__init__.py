@@ -109,7 +109,7 @@ def load(path=None):
         "blsgpu_set_batched_engine": ([i32, i32, i32, i32, i32], i32),
         "blsgpu_get_batched_engine": ([i32, vp], i32),
         "blsgpu_batched_stats": ([vp], i32),
-        "blsgpu_autotune_device": ([vp, vp, vp, i32, vp, vp, vp], i32),
+        "blsgpu_autotune_device": ([vp, vp, vp, i32, i32, vp, vp, vp], i32),
         "blsgpu_server_create": ([i32, i32, i64, i32, i32], vp),
         "blsgpu_server_step_capacity": ([vp], i32),
         "blsgpu_server_submit": ([vp, vp, vp, vp, i32], i64),

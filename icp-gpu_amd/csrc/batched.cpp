@@ -310,10 +310,10 @@ int blsgpu_batched_stats(int out[2]) {
 }
 
 int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx, int M,
-                           void* stream, int cfg_out[4], double ms_out[2]) {
+                           int form, void* stream, int cfg_out[4], double ms_out[2]) {
   std::lock_guard<std::mutex> lk(g.mu);
   g_err.clear();
-  if (!d_sigs48 || !d_neg_hms || !d_key_idx || M <= 0) {
+  if (!d_sigs48 || !d_neg_hms || !d_key_idx || M <= 0 || (form != 0 && form != 1)) {
     g_err = "invalid arguments";
     return -1;
   }
@@ -355,7 +355,7 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
   const int gml[] = {8, 12, 13, 16, 32}, gfe[] = {12, 16, 18, 32};
   double best_ml = 1e30, best_fe = 1e30;
   for (int G : gml) {
-    const int occ = bls::occupancy_grp(0, G);
+    const int occ = form == 0 ? bls::occupancy_grp(0, G) : -1;
     for (int w = -1; w <= occ; w++) {
       if (w == 0) continue;  // the same grid as w == occ
       a.g_ml = G;
@@ -373,7 +373,7 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
   a.wpc_ml = c.wpc_ml;
   CHK(bls::engine_miller(a, g.s));  // the final exponentiation candidates read this output
   for (int G : gfe) {
-    const int occ = bls::occupancy_grp(1, G);
+    const int occ = form == 0 ? bls::occupancy_grp(1, G) : -1;
     for (int w = -1; w <= occ; w++) {
       if (w == 0) continue;
       a.g_fe = G;
@@ -387,7 +387,13 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
       }
     }
   }
-  g.tuned[size_class(M)] = c;
+  if (form == 0) {
+    g.tuned[size_class(M)] = c;
+  } else {
+    g.mode = 1;
+    g.g_ml = c.g_ml;
+    g.g_fe = c.g_fe;
+  }
   if (cfg_out) {
     cfg_out[0] = c.g_ml;
     cfg_out[1] = c.g_fe;

@@ -333,115 +333,126 @@ __device__ __forceinline__ Fp inv(const Fp& a) { return pow_fixed(a, INV_EXP); }
 // a^((p+1)/4); ok = (result^2 == a) (ic-integration/gpu_crypto/cuda/decompress.cu:16-40)
 __device__ __forceinline__ Fp sqrt_cand(const Fp& a) { return pow_fixed(a, SQRT_EXP); }
 
-// ---- inverse by the binary extended Euclidean algorithm ----
-// Variable time (every input here is public).  Works on the canonical 384-bit value
-// c = a R' mod p: u = c, v = p, invariants x1 c = u and x2 c = v (mod p); halving u or v
-// halves the matching x (adding p first when odd); the larger of u, v absorbs the
-// difference.  At u == 1 (or v == 1), x1 (or x2) is c^-1; the Montgomery form of a^-1
-// is c^-1 R'^2 = mont(c^-1, R'^3).  The reference inverts by Fermat exponentiation
+// ---- inverse by batched divsteps (Bernstein-Yang "safegcd", eprint 2019/266) ----
+// Variable time (every input here is public).  Works on the canonical value
+// c = a R' mod p, held in the same 14-limb radix-2^28 form as Fp: f = p, g = c, and
+// d, e with d c = f, e c = g (mod p).  Each batch runs 28 divsteps on the low 28 bits
+// of f and g alone (the decisions depend on nothing else), which yields a 2x2 matrix
+// T with entries of magnitude <= 2^28; then (f, g) = T (f, g) / 2^28 exactly, and
+// (d, e) = T (d, e) / 2^28 mod p, the division made exact by adding m p with
+// m = -(T (d, e))_0 p^-1 mod 2^28.  g reaches 0 after about 29 batches for 381-bit
+// inputs; then f = +-1 and d f = c^-1.  The Montgomery form of a^-1 is
+// c^-1 R'^2 = mont(c^-1, R'^3).  All limb products are < 2^56 and every column sum
+// fits an i64.  The reference inverts by Fermat exponentiation
 // (gpu-crypto/cuda/field_sppark.cuh:40-50); the result is the same field element.
-__device__ __forceinline__ bool w12_is_one(const u32 x[12]) {
-  u32 acc = x[0] ^ 1u;
-#pragma unroll
-  for (int j = 1; j < 12; j++) acc |= x[j];
-  return acc == 0;
-}
-__device__ __forceinline__ void w12_shr1(u32 x[12], u32 top) {
-#pragma unroll
-  for (int j = 0; j < 11; j++) x[j] = (x[j] >> 1) | (x[j + 1] << 31);
-  x[11] = (x[11] >> 1) | (top << 31);
-}
-// x = x / 2 mod p  (x < p)
-__device__ __forceinline__ void w12_half_mod(u32 x[12]) {
-  if (x[0] & 1u) {
-    u64 c = 0;
-#pragma unroll
-    for (int j = 0; j < 12; j++) {
-      c += (u64)x[j] + P32[j];
-      x[j] = (u32)c;
-      c >>= 32;
+constexpr u32 P_INV28 = 0x30003;  // p^-1 mod 2^28
+static_assert(((u64)(u32)P_L[0] * P_INV28) % (1u << 28) == 1, "P_INV28 is not p^-1 mod 2^28");
+
+// 28 divsteps on the low bits; eta = -delta.  Zeros of g are skipped in bulk; an odd
+// g takes one step that either swaps (f, g) <- (g, -f) or not and then adds f to g.
+__device__ __forceinline__ void sg_divsteps(int& eta, u32 f, u32 g, i32& u, i32& v, i32& q, i32& r) {
+  u = 1;
+  v = 0;
+  q = 0;
+  r = 1;
+  int i = 28;
+  for (int guard = 0; guard < 64; guard++) {
+    const int z = __builtin_ctz(g | (0xffffffffu << i));  // <= i
+    g >>= z;
+    u <<= z;
+    v <<= z;
+    eta -= z;
+    i -= z;
+    if (i == 0) break;
+    if (eta < 0) {
+      eta = -eta;
+      const u32 t = f;
+      f = g;
+      g = 0u - t;
+      const i32 tu = u, tv = v;
+      u = q;
+      v = r;
+      q = -tu;
+      r = -tv;
     }
-    w12_shr1(x, (u32)c);
-  } else {
-    w12_shr1(x, 0u);
+    g += f;
+    q += u;
+    r += v;
   }
 }
-__device__ __forceinline__ bool w12_geq(const u32 a[12], const u32 b[12]) {
-  bool ge = true, decided = false;
-#pragma unroll
-  for (int j = 11; j >= 0; j--) {
-    if (!decided && a[j] != b[j]) {
-      ge = a[j] > b[j];
-      decided = true;
-    }
+
+// (x, y) <- (a x + b y, c x + d y) / 2^28 for normalized limb vectors (limbs 0..12 in
+// [0, 2^28), signed top); with_p adds m p to each row first so the division is exact.
+__device__ __forceinline__ void sg_update(i32 (&x)[NL], i32 (&y)[NL], i32 a, i32 b, i32 c, i32 d, bool with_p) {
+  i64 cx = (i64)a * x[0] + (i64)b * y[0];
+  i64 cy = (i64)c * x[0] + (i64)d * y[0];
+  i32 mx = 0, my = 0;
+  if (with_p) {
+    mx = (i32)((0u - (u32)cx) * P_INV28 & (u32)MASK28);
+    my = (i32)((0u - (u32)cy) * P_INV28 & (u32)MASK28);
+    if (mx >= (1 << 27)) mx -= 1 << 28;
+    if (my >= (1 << 27)) my -= 1 << 28;
+    cx += (i64)mx * P_L[0];
+    cy += (i64)my * P_L[0];
   }
-  return ge;
+  cx >>= 28;
+  cy >>= 28;
+#pragma unroll
+  for (int j = 1; j < NL; j++) {
+    cx += (i64)a * x[j] + (i64)b * y[j];
+    cy += (i64)c * x[j] + (i64)d * y[j];
+    if (with_p) {
+      cx += (i64)mx * P_L[j];
+      cy += (i64)my * P_L[j];
+    }
+    x[j - 1] = (i32)cx & MASK28;
+    y[j - 1] = (i32)cy & MASK28;
+    cx >>= 28;
+    cy >>= 28;
+  }
+  x[NL - 1] = (i32)cx;
+  y[NL - 1] = (i32)cy;
 }
-// a -= b (a >= b)
-__device__ __forceinline__ void w12_sub(u32 a[12], const u32 b[12]) {
-  i64 c = 0;
+
+// 0 maps to 0.  UNIFORM: the caller guarantees that every active lane holds the same
+// value (in practice, one active lane); the divstep decisions then run on the scalar
+// unit, with scalar branches, from values read out of the first active lane.
+template <bool UNIFORM>
+__device__ __noinline__ Fp inv_fp_t(const Fp& a) {
+  const Fp c = canon(a);
+  i32 f[NL], g[NL], d[NL], e[NL];
 #pragma unroll
-  for (int j = 0; j < 12; j++) {
-    c += (i64)a[j] - b[j];
-    a[j] = (u32)c;
-    c >>= 32;
+  for (int j = 0; j < NL; j++) {
+    f[j] = P_L[j];
+    g[j] = c.l[j];
+    d[j] = 0;
+    e[j] = j == 0;
   }
+  int eta = -1;
+  for (int batch = 0; batch < 64; batch++) {  // about 29 are needed; the cap only bounds the loop
+    i32 any = 0;
+#pragma unroll
+    for (int j = 0; j < NL; j++) any |= g[j];
+    u32 f0 = (u32)f[0], g0 = (u32)g[0];
+    if (UNIFORM) {
+      any = __builtin_amdgcn_readfirstlane(any);
+      f0 = __builtin_amdgcn_readfirstlane(f0);
+      g0 = __builtin_amdgcn_readfirstlane(g0);
+    }
+    if (any == 0) break;
+    i32 u, v, q, r;
+    sg_divsteps(eta, f0, g0, u, v, q, r);
+    sg_update(f, g, u, v, q, r, false);
+    sg_update(d, e, u, v, q, r, true);
+  }
+  Fp x;
+  const bool negf = f[NL - 1] < 0;  // f = -1
+#pragma unroll
+  for (int j = 0; j < NL; j++) x.l[j] = negf ? -d[j] : d[j];
+  return mul(x, fp_const(C_R3_INT));
 }
-// a = a - b mod p  (a, b < p)
-__device__ __forceinline__ void w12_sub_mod(u32 a[12], const u32 b[12]) {
-  i64 c = 0;
-#pragma unroll
-  for (int j = 0; j < 12; j++) {
-    c += (i64)a[j] - b[j];
-    a[j] = (u32)c;
-    c >>= 32;
-  }
-  if (c < 0) {
-    u64 d = 0;
-#pragma unroll
-    for (int j = 0; j < 12; j++) {
-      d += (u64)a[j] + P32[j];
-      a[j] = (u32)d;
-      d >>= 32;
-    }
-  }
-}
-// 0 maps to 0
-__device__ __noinline__ Fp inv_bgcd(const Fp& a) {
-  u32 u[12], v[12], x1[12], x2[12];
-  pack384(canon(a), u);
-  u32 nz = 0;
-#pragma unroll
-  for (int j = 0; j < 12; j++) {
-    nz |= u[j];
-    v[j] = P32[j];
-    x1[j] = j == 0;
-    x2[j] = 0;
-  }
-  if (nz == 0) return fp_zero();
-  while (!w12_is_one(u) && !w12_is_one(v)) {
-    while ((u[0] & 1u) == 0) {
-      w12_shr1(u, 0u);
-      w12_half_mod(x1);
-    }
-    while ((v[0] & 1u) == 0) {
-      w12_shr1(v, 0u);
-      w12_half_mod(x2);
-    }
-    if (w12_geq(u, v)) {
-      w12_sub(u, v);
-      w12_sub_mod(x1, x2);
-    } else {
-      w12_sub(v, u);
-      w12_sub_mod(x2, x1);
-    }
-  }
-  const u32* r = w12_is_one(u) ? x1 : x2;
-  u32 rr[12];
-#pragma unroll
-  for (int j = 0; j < 12; j++) rr[j] = r[j];
-  return mul(unpack384(rr), fp_const(C_R3_INT));
-}
+__device__ __forceinline__ Fp inv_fp(const Fp& a) { return inv_fp_t<false>(a); }
+__device__ __forceinline__ Fp inv_fp_uniform(const Fp& a) { return inv_fp_t<true>(a); }
 
 // canonical plain value > (p-1)/2 (decompress.cu:107-135)
 __device__ __forceinline__ bool lex_largest(const Fp& a) {

@@ -7,7 +7,9 @@
 // enough for the instruction cache and needs no calls or scratch.
 //
 // Workgroups are one wavefront (64 lanes) holding 64 / G items; lanes past the last
-// full group idle but take part in the barriers.
+// full group idle but take part in the barriers.  The Miller value passes between the
+// two kernels item-major (168 consecutive words per item), so the G lanes of an item
+// read and write whole cache lines whichever XCD runs the workgroup.
 #include <hip/hip_runtime.h>
 
 #include "curve.hpp"
@@ -240,7 +242,7 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
   if (valid && act) {
     for (int w = sub; w < 12 * NL; w += G) {
       const int e = w / NL, l = w % NL;
-      fout[(size_t)w * M + item] = g_lds[F + e * SLOT + l];
+      fout[(size_t)item * (12 * NL) + w] = g_lds[F + e * SLOT + l];
     }
   }
   __syncthreads();  // the next tile reuses this tile's LDS
@@ -319,7 +321,7 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
   if (in_wg) {
     for (int w = sub; w < 12 * NL; w += G) {
       const int e = w / NL, l = w % NL;
-      g_lds[R0 + e * SLOT + l] = act ? fin[(size_t)w * M + it] : 0;
+      g_lds[R0 + e * SLOT + l] = act ? fin[(size_t)it * (12 * NL) + w] : 0;
     }
   }
   __syncthreads();
@@ -331,7 +333,8 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
   run_op<G>(sched::OP_N2, sub, act, R3, 0, R3 + 2 * SLOT, T, K);  // R3[2] = n
   // one inversion per workgroup (Montgomery's trick over its IPW items): thread 0 forms
   // the prefix products of the n's (inactive or zero entries count as 1), inverts the
-  // total by binary GCD and peels the individual inverses off into R3[3] of each item
+  // total by batched divsteps (inv_fp_uniform) and peels the individual inverses off
+  // into R3[3] of each item
   if (threadIdx.x == 0) {
     Fp pre[IPW];
     Fp accp = fp_one();
@@ -342,7 +345,7 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
       pre[q] = accp;
       accp = mul(accp, nq);
     }
-    Fp inv_all = inv_bgcd(accp);
+    Fp inv_all = inv_fp_uniform(accp);  // thread 0 alone
     for (int q = IPW - 1; q >= 0; q--) {
       const int bq = q * FE_SLOTS * SLOT + 36 * SLOT;
       Fp nq = lds_ld(bq + 2 * SLOT);

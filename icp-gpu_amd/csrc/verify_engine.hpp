@@ -25,7 +25,7 @@ struct EngineArgs {
   int32_t* pk_lines = nullptr;  // M x 5712 (legacy)
   int32_t* status = nullptr;    // M (fused): decode status
   int32_t* sub_status = nullptr; // M (fused): subgroup status
-  int32_t* fbuf = nullptr;      // M x 168 Miller values
+  int32_t* fbuf = nullptr;      // M x 168 Miller values: word-major (SoA) in mode 0, item-major in mode 1
   // outputs
   int32_t* results = nullptr;  // M
   uint64_t* gt = nullptr;      // M x 72 (nullable)

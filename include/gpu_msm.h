@@ -136,14 +136,17 @@ int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int
 int blsgpu_get_batched_engine(int M, int cfg[5]);
 /* out[0] graphs instantiated, out[1] graph launches */
 int blsgpu_batched_stats(int out[2]);
-/* Autotuner: times every lanes-per-item and workgroups-per-CU candidate of the Miller
- * loop, then of the final exponentiation, on the given device-resident job (M items),
- * and records the fastest pair for M's size class (ceil(log2 M)); later batched calls
- * and serving steps use the entry of the nearest class.  cfg_out (nullable) =
- * {g_miller, g_final_exp, wpc_miller, wpc_final_exp}; ms_out (nullable) = the two best
- * kernel times in ms.  Synchronous. */
+/* Autotuner on a device-resident job of M items, timed with HIP events (best of 2).
+ * form 0 (batched): times every lanes-per-item and workgroups-per-CU candidate of the
+ *   Miller loop, then of the final exponentiation, and records the fastest pair for M's
+ *   size class (ceil(log2 M)); later batched calls and serving steps use the entry of
+ *   the nearest class.
+ * form 1 (plain): times the lanes-per-item candidates with one workgroup per tile and
+ *   makes the fastest pair the plain form's engine (as blsgpu_set_engine(1, ...)).
+ * cfg_out (nullable) = {g_miller, g_final_exp, wpc_miller, wpc_final_exp}; ms_out
+ * (nullable) = the two best kernel times in ms.  Synchronous. */
 int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx, int M,
-                           void* stream, int cfg_out[4], double ms_out[2]);
+                           int form, void* stream, int cfg_out[4], double ms_out[2]);
 
 /* Engine selection: mode 0 = one thread per verification (thread-per-item kernels),
  * mode 1 = a group of G lanes of one wavefront per verification (g_miller in

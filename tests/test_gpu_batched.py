@@ -134,7 +134,7 @@ def test_autotuner_picks_a_valid_config_and_stays_exact(small):
     sig, nhm, ki, res, gt = _device_run(small, h, None)
     cfg = np.zeros(4, np.int32)
     ms = np.zeros(2, np.float64)
-    rc = blsgpu.load().blsgpu_autotune_device(sig.data_ptr(), nhm.data_ptr(), ki.data_ptr(), 97,
+    rc = blsgpu.load().blsgpu_autotune_device(sig.data_ptr(), nhm.data_ptr(), ki.data_ptr(), 97, 0,
                                               torch.cuda.current_stream().cuda_stream, O.ptr(cfg), O.ptr(ms))
     assert rc == 0, blsgpu.last_error()
     assert cfg[0] in (8, 12, 13, 16, 32) and cfg[1] in (12, 16, 18, 32)
@@ -144,3 +144,13 @@ def test_autotuner_picks_a_valid_config_and_stays_exact(small):
     assert (res2 == small["want"]).all()
     assert (gt2 == small["wgt"]).all()
     blsgpu.set_batched_engine(12, 16, 0, 0, True)
+    # plain form: sets the plain engine's lanes per item
+    saved = blsgpu.get_engine()
+    rc = blsgpu.load().blsgpu_autotune_device(sig.data_ptr(), nhm.data_ptr(), ki.data_ptr(), 97, 1,
+                                              torch.cuda.current_stream().cuda_stream, O.ptr(cfg), O.ptr(ms))
+    assert rc == 0, blsgpu.last_error()
+    assert cfg[2] == -1 and cfg[3] == -1
+    assert blsgpu.get_engine() == (1, int(cfg[0]), int(cfg[1]))
+    r3, g3 = blsgpu.verify_compressed(small["sig"], small["nhm"], h[small["key_idx"]], want_gt=True)
+    assert (r3 == small["want"]).all() and (g3 == small["wgt"]).all()
+    blsgpu.set_engine(*saved)
