@@ -85,17 +85,12 @@ __device__ __forceinline__ Fp dbl(const Fp& a) { return add(a, a); }
 // negative limbs borrow correctly); the top limb t absorbs the carry, |t| < 2^31.
 // Fold: q = floor(t * 40323 / 2^32) ~ t / 106513 where 106513 = P_L[13]; subtract
 // q*p.  The top becomes t - q*P13 - borrow with |q| < 2^14.3, so |top| < 2^18.4.
-__device__ __forceinline__ Fp norm(const Fp& a) {
-  Fp r;
-  i32 c = 0;
-#pragma unroll
-  for (int i = 0; i < NL - 1; i++) {
-    i32 v = a.l[i] + c;
-    r.l[i] = v & MASK28;
-    c = v >> 28;
-  }
-  i32 t = a.l[NL - 1] + c;
-  i32 q = (i32)(((i64)t * 40323) >> 32);
+// fold_top() is the fold alone, for a value already carried (limbs 0..12 in
+// [0, 2^28), |top| < 2^31); norm() is carry pass + fold.
+__device__ __forceinline__ Fp fold_top(const Fp& a) {
+  Fp r = a;
+  const i32 t = a.l[NL - 1];
+  const i32 q = (i32)(((i64)t * 40323) >> 32);
   i64 acc = 0;
 #pragma unroll
   for (int i = 0; i < NL - 1; i++) {
@@ -105,6 +100,18 @@ __device__ __forceinline__ Fp norm(const Fp& a) {
   }
   r.l[NL - 1] = t - q * P_L[NL - 1] + (i32)acc;
   return r;
+}
+__device__ __forceinline__ Fp norm(const Fp& a) {
+  Fp r;
+  i32 c = 0;
+#pragma unroll
+  for (int i = 0; i < NL - 1; i++) {
+    i32 v = a.l[i] + c;
+    r.l[i] = v & MASK28;
+    c = v >> 28;
+  }
+  r.l[NL - 1] = a.l[NL - 1] + c;
+  return fold_top(r);
 }
 
 // Carry pass only: limbs 0..12 -> [0, 2^28), value unchanged (top limb absorbs the

@@ -442,106 +442,176 @@ OPS = [
 ]
 
 
-def term_byte(region, idx):
-    assert 0 <= idx < 64
-    return (region << 6) | idx
+# ---- per-G round tables ----
+# Each op runs as rounds.  In a product round every lane of an item's group computes
+# at most one product: operand A = sum of its A terms, B = sum of its B terms, each term
+# a (slot, small signed coefficient) pair.  The number of terms is the same on every
+# lane of a round (shorter lists are padded with zero-coefficient terms), so the kernel
+# executes one instruction stream with no divergence.  Output rounds likewise compute
+# one output per lane as a sum of terms over products, inputs and constants.
+#
+# Term (16 bits): bits 0..5 slot index in its region, bits 6..7 region (A, B, T, K),
+# bits 8..15 the coefficient as a signed byte.  A lane's record of one round is a whole
+# number of 16-byte units: product rounds [dst, A terms (8 places), B terms (8 places)]
+# padded to 24 halfwords; output rounds [dst, terms] padded to a multiple of 8.
+# dst = 0xffff marks an idle lane.  Lane s of the group reads the record at
+# desc + s * q (in 16-byte units).
+GS = [8, 9, 12, 13, 16, 18, 20, 27, 32]
+IDLE = 0xFFFF
+PROD_Q = 3        # 24 halfwords per product record
+B_AT = 9          # first B term of a product record
+MAX_OPERAND_TERMS = 8
+MAX_OUT_Q = 5     # output records of at most 39 terms
 
 
-def encode_terms(lin):
-    """(n_pos, n_neg, term bytes): positive terms then negative terms, a coefficient c
-    expanded into |c| copies of its term"""
-    pos, neg = [], []
-    for (region, idx), c in sorted(lin.t.items()):
-        assert -8 <= c <= 8, c
-        (pos if c > 0 else neg).extend([term_byte(region, idx)] * abs(c))
-    return len(pos), len(neg), pos + neg
+def term16(region, idx, coef):
+    assert 0 <= idx < 64 and -128 <= coef <= 127 and coef != 0
+    return (region << 6) | idx | ((coef & 0xFF) << 8)
+
+
+def lin_terms(lin):
+    return [term16(r, i, c) for (r, i), c in sorted(lin.t.items())]
+
+
+def weight(lin):
+    return sum(abs(c) for c in lin.t.values())
+
+
+def op_programs():
+    """[(name, products [(A Lin, B Lin)], outputs [Lin])], products in T order, with
+    the operand-weight bounds the kernel's arithmetic relies on checked"""
+    progs = []
+    for name, fn in OPS:
+        R, outs = fn()
+        for a, b in R.products:
+            # operands: at most 8 terms and weight 16 (|value| < 2^386.4 keeps the
+            # Montgomery product N1); outputs: weight < 2^12 (64-bit accumulation)
+            assert len(a.t) <= MAX_OPERAND_TERMS and len(b.t) <= MAX_OPERAND_TERMS, name
+            assert weight(a) <= 16 and weight(b) <= 16, name
+        for o in outs:
+            assert weight(o) < 4096 and len(o.t) <= 8 * MAX_OUT_Q - 1, name
+        # hazard check: an output that reads input A index k other than its own index
+        # would see a value overwritten in place; the kernels write such outputs to a
+        # distinct register.
+        reads_other = any(any(r == A and idx != i for (r, idx) in o.t) for i, o in enumerate(outs))
+        progs.append((name, list(R.products), outs, reads_other))
+    return progs
+
+
+def build_rounds(G, progs):
+    """per-op round lists for a group of G lanes:
+    [(name, prod_rounds, out_rounds)], a round = (na, nb, [per-lane record halfwords])"""
+    ops = []
+    for name, prods, outs, _ in progs:
+        order = sorted(range(len(prods)), key=lambda j: (len(prods[j][0].t) + len(prods[j][1].t),
+                                                         len(prods[j][0].t)))
+        prounds = []
+        for r0 in range(0, len(order), G):
+            chunk = order[r0:r0 + G]
+            na = max(len(prods[j][0].t) for j in chunk)
+            nb = max(len(prods[j][1].t) for j in chunk)
+            recs = []
+            for s in range(G):
+                rec = [0] * (8 * PROD_Q)
+                rec[0] = IDLE
+                if s < len(chunk):
+                    j = chunk[s]
+                    rec[0] = j
+                    ta, tb = lin_terms(prods[j][0]), lin_terms(prods[j][1])
+                    rec[1:1 + len(ta)] = ta
+                    rec[B_AT:B_AT + len(tb)] = tb
+                recs.append(rec)
+            prounds.append((na, nb, recs))
+        oorder = sorted(range(len(outs)), key=lambda i: len(outs[i].t))
+        orounds = []
+        for r0 in range(0, len(oorder), G):
+            chunk = oorder[r0:r0 + G]
+            n = max(len(outs[i].t) for i in chunk)
+            q = (1 + n + 7) // 8
+            recs = []
+            for s in range(G):
+                rec = [0] * (8 * q)
+                rec[0] = IDLE
+                if s < len(chunk):
+                    i = chunk[s]
+                    rec[0] = i
+                    t = lin_terms(outs[i])
+                    rec[1:1 + len(t)] = t
+                recs.append(rec)
+            orounds.append((n, 0, recs))
+        ops.append((name, prounds, orounds))
+    return ops
 
 
 def limbs28(v):
     return [(v >> (28 * i)) & ((1 << 28) - 1) for i in range(14)]
 
 
-def main():
-    code = []
-    ops_meta = []
-    max_t = 0
-    max_terms = 0
-    for name, fn in OPS:
-        R, outs = fn()
-        # products are independent: order them by operand term count so that the lanes
-        # of one round (products j = r*G .. r*G+G-1) do similar amounts of gathering;
-        # outputs carry their destination index, so they can be ordered the same way
-        order = sorted(range(len(R.products)),
-                       key=lambda j: sum(abs(c) for c in R.products[j][0].t.values()) +
-                       sum(abs(c) for c in R.products[j][1].t.values()))
-        remap = {old: new for new, old in enumerate(order)}
-        R.products = [R.products[j] for j in order]
+def emit_tables(G, ops):
+    """C++ arrays of one G: OPS<G>, ROUNDS<G>, DESC<G>"""
+    op_rows, round_rows, words = [], [], []
+    for name, prounds, orounds in ops:
+        op_rows.append("{%d, %d, %d}" % (len(round_rows), len(prounds), len(orounds)))
+        for na, nb, recs in prounds + orounds:
+            q = len(recs[0]) // 8
+            round_rows.append("{%d, %d, %d, %d, 0}" % (len(words) // 4, na, nb, q))
+            for rec in recs:
+                for k in range(0, len(rec), 2):
+                    words.append(rec[k] | (rec[k + 1] << 16))
+    assert len(round_rows) < 65536
+    out = ["// lanes per item G = %d: %d rounds, %d bytes of records" % (G, len(round_rows), 4 * len(words)),
+           "__constant__ const OpRounds OPS%d[NUM_OPS] = {%s};" % (G, ", ".join(op_rows)),
+           "__constant__ const Round ROUNDS%d[%d] = {%s};" % (G, len(round_rows), ", ".join(round_rows)),
+           "__device__ __align__(16) const uint32_t DESC%d[%d] = {%s};" % (
+               G, len(words), ", ".join("0x%x" % w for w in words)),
+           "template <> struct Tab<%d> {" % G,
+           "  static __device__ __forceinline__ const COpRounds* ops() { return (const COpRounds*)OPS%d; }" % G,
+           "  static __device__ __forceinline__ const CRound* rounds() { return (const CRound*)ROUNDS%d; }" % G,
+           "  static __device__ __forceinline__ const uint4* desc() { return reinterpret_cast<const uint4*>(DESC%d); }"
+           % G,
+           "};"]
+    return out, 4 * len(words)
 
-        def ren(lin):
-            return Lin({((T, remap[i]) if r == T else (r, i)): c for (r, i), c in lin.t.items()})
-        outs = [ren(o) for o in outs]
-        out_order = sorted(range(len(outs)), key=lambda i: sum(abs(c) for c in outs[i].t.values()))
-        p_start, o_start = [], []
-        for j, (a, b) in enumerate(R.products):
-            ap, an, ea = encode_terms(a)
-            bp, bn, eb = encode_terms(b)
-            max_terms = max(max_terms, len(ea), len(eb))
-            p_start.append(len(code))
-            code += [j, ap, an, bp, bn] + ea + eb
-        for i in out_order:
-            # output i is written to index i of the destination register
-            o = outs[i]
-            op_, on, eo = encode_terms(o)
-            max_terms = max(max_terms, len(eo))
-            o_start.append(len(code))
-            code += [i, op_, on] + eo
-        # hazard check: an output that reads input A index k other than its own index
-        # would see a value overwritten in place; the kernels write outputs to a
-        # distinct register whenever this is flagged.
-        reads_other = any(any(r == A and idx != i for (r, idx) in o.t) for i, o in enumerate(outs))
-        max_t = max(max_t, len(R.products))
-        ops_meta.append((name, len(R.products), len(outs), p_start, o_start, reads_other))
-    assert len(code) < 65536
+
+def main():
+    progs = op_programs()
     lines = ["// Generated by gen_schedules.py -- do not edit.",
-             "// Bilinear schedules of the Fp12 operations (see gen_schedules.py for the encoding).",
-             "#pragma once", "#include <stdint.h>", "namespace bls {", "namespace sched {"]
-    for k, (name, np_, no, _, _, ro) in enumerate(ops_meta):
+             "// Fp12-level operations as per-G round tables (see gen_schedules.py for the encoding).",
+             "#pragma once", "#include <hip/hip_runtime.h>", "#include <stdint.h>", "namespace bls {",
+             "namespace sched {"]
+    for k, (name, prods, outs, ro) in enumerate(progs):
         lines.append("constexpr int %s = %d;  // %d products, %d outputs%s" %
-                     (name, k, np_, no, ", outputs read other inputs" if ro else ""))
-    lines.append("constexpr int NUM_OPS = %d;" % len(ops_meta))
-    lines.append("constexpr int MAX_PRODUCTS = %d;" % max_t)
-    lines.append("constexpr int MAX_TERMS = %d;" % max_terms)
+                     (name, k, len(prods), len(outs), ", outputs read other inputs" if ro else ""))
+    lines.append("constexpr int NUM_OPS = %d;" % len(progs))
     lines.append("constexpr int NUM_CONSTS = %d;" % len(CONSTS.vals))
-    lines.append("constexpr uint16_t NPROD[NUM_OPS] = {%s};" % ", ".join(str(m[1]) for m in ops_meta))
-    lines.append("constexpr uint16_t NOUT[NUM_OPS] = {%s};" % ", ".join(str(m[2]) for m in ops_meta))
-    starts, offs = [], []
-    for m in ops_meta:
-        offs.append(len(starts))
-        starts += m[3]
-        starts += m[4]
-    lines.append("constexpr uint16_t START_OFF[NUM_OPS] = {%s};  // index into START: products then outputs" %
-                 ", ".join(str(o) for o in offs))
-    lines.append("constexpr int START_LEN = %d;" % len(starts))
-    starts_p = starts + [0] * (len(starts) % 2)
-    lines.append("__device__ __align__(4) const uint16_t START[%d] = {%s};" % (len(starts_p), ", ".join(map(str, starts_p))))
-    lines.append("constexpr int CODE_LEN = %d;" % len(code))
-    body = ", ".join(str(c) for c in code)
-    pad = (-len(code)) % 4
-    body = ", ".join(str(c) for c in code + [0] * pad)
-    lines.append("__device__ __align__(4) const uint8_t CODE[%d] = {%s};" % (len(code) + pad, body))
+    lines.append("constexpr int PROD_Q = %d;  // 16-byte units per product record" % PROD_Q)
+    lines.append("constexpr int B_AT = %d;    // halfword index of the first B term" % B_AT)
+    lines.append("constexpr int MAX_OUT_Q = %d;" % MAX_OUT_Q)
+    lines.append("constexpr uint16_t IDLE = 0x%x;" % IDLE)
+    lines.append("struct OpRounds { uint16_t first; uint8_t np, no; };")
+    lines.append("struct Round { uint32_t desc; uint8_t na, nb, q, pad; };")
+    lines.append("// the op and round tables sit in the constant address space, so that loads at")
+    lines.append("// uniform indices are scalar loads")
+    lines.append("typedef const OpRounds __attribute__((address_space(4))) COpRounds;")
+    lines.append("typedef const Round __attribute__((address_space(4))) CRound;")
+    lines.append("template <int G> struct Tab;")
     lines.append("// constants (internal Montgomery form, 28-bit limbs)")
-    kvals = []
-    for v in CONSTS.vals:
-        kvals.append("{%s}" % ", ".join("0x%07x" % x for x in limbs28(v * RI % P)))
-    if not kvals:
-        kvals = ["{0}"]
+    kvals = ["{%s}" % ", ".join("0x%07x" % x for x in limbs28(v * RI % P)) for v in CONSTS.vals] or ["{0}"]
     lines.append("__device__ const int32_t KCONST[%d][14] = {%s};" % (max(1, len(CONSTS.vals)), ", ".join(kvals)))
+    sizes = {}
+    for G in GS:
+        ops = build_rounds(G, progs)
+        t, nbytes = emit_tables(G, ops)
+        lines += t
+        sizes[G] = (nbytes, sum(len(p) + len(o) for _, p, o in ops))
     lines += ["}  // namespace sched", "}  // namespace bls"]
     with open(os.path.join(HERE, "schedules_gen.hpp"), "w") as f:
         f.write("\n".join(lines) + "\n")
-    for m in ops_meta:
-        print("%-12s products %3d outputs %3d%s" % (m[0], m[1], m[2], "  (reads other inputs)" if m[5] else ""))
-    print("constants", len(CONSTS.vals), "max terms", max_terms, "code bytes", len(code))
+    for name, prods, outs, ro in progs:
+        print("%-12s products %3d outputs %3d%s" % (name, len(prods), len(outs), "  (reads other inputs)" if ro else ""))
+    for G, (nb, nr) in sizes.items():
+        print("G=%2d: %4d rounds, %6d bytes of records" % (G, nr, nb))
+    print("constants", len(CONSTS.vals))
 
 
 if __name__ == "__main__":

@@ -54,96 +54,141 @@ __device__ __forceinline__ void st_ref_words(u64* p, const Fp& a) {
 struct Regions {
   int a, b, t, k;
   __device__ __forceinline__ int base(int reg) const { return reg == 0 ? a : reg == 1 ? b : reg == 2 ? t : k; }
+  // the same, branch-free (reg differs between lanes)
+  __device__ __forceinline__ int select(u32 reg) const {
+    const int m1 = -(int)(reg & 1), m2 = -(int)((reg >> 1) & 1);
+    const int lo = a ^ ((a ^ b) & m1), hi = t ^ ((t ^ k) & m1);
+    return lo ^ ((lo ^ hi) & m2);
+  }
 };
 
-// The schedule tables are copied into LDS at kernel start (byte offset SCHED_B of
-// g_lds for CODE, START after it), so that term decoding costs LDS latency only.
-constexpr int CODE_WORDS = (sched::CODE_LEN + 3) / 4;
-constexpr int START_WORDS = (sched::START_LEN + 1) / 2;
-constexpr int SCHED_WORDS = CODE_WORDS + START_WORDS;
+// ---- the round interpreter ----
+// A lane's record of one round is held as 32-bit words (two halfword terms each).
+constexpr int REC_WORDS = 4 * sched::MAX_OUT_Q;
+static_assert(sched::PROD_Q <= sched::MAX_OUT_Q, "record buffer");
 
-__device__ __forceinline__ void load_schedule(int sched_w) {
-  const uint32_t* code = reinterpret_cast<const uint32_t*>(sched::CODE);
-  const uint32_t* start = reinterpret_cast<const uint32_t*>(sched::START);
-  for (int w = threadIdx.x; w < SCHED_WORDS; w += blockDim.x)
-    g_lds[sched_w + w] = (i32)(w < CODE_WORDS ? code[w] : start[w - CODE_WORDS]);
+__device__ __forceinline__ u32 rec_hw(const u32 (&rec)[REC_WORDS], int k) {
+  return (k & 1) ? (rec[k >> 1] >> 16) : (rec[k >> 1] & 0xffffu);
 }
-__device__ __forceinline__ int code_byte(int sched_w, int idx) {
-  return reinterpret_cast<const uint8_t*>(g_lds + sched_w)[idx];
-}
-__device__ __forceinline__ int start_of(int sched_w, int idx) {
-  return reinterpret_cast<const uint16_t*>(g_lds + sched_w + CODE_WORDS)[idx];
-}
-
-// acc +/- sum of n terms starting at code byte `at`, in chunks of up to 6 terms whose
-// loads are all issued before their additions.  Every slot value has |limb| < 2^28 and a
-// carry pass whenever the accumulated weight would pass 7 keeps every limb below 2^31.
-template <bool SUB>
-__device__ __forceinline__ void gather_sign(Fp& acc, int& w, int sched_w, int at, int n, const Regions& R) {
-  for (int t = 0; t < n; t += 6) {
-    const int k = n - t < 6 ? n - t : 6;
-    Fp x[6];
+// this lane's record of a round: q 16-byte units at desc + sub * q (q is uniform)
+__device__ __forceinline__ void load_record(u32 (&rec)[REC_WORDS], const uint4* __restrict__ desc, u32 at, int q,
+                                            int sub) {
+  const uint4* p = desc + at + (u32)sub * q;
 #pragma unroll
-    for (int j = 0; j < 6; j++) {
-      if (j < k) {
-        const int tb = code_byte(sched_w, at + t + j);
-        x[j] = lds_ld(R.base(tb >> 6) + (tb & 63) * SLOT);
-      }
+  for (int u = 0; u < sched::MAX_OUT_Q; u++) {
+    if (u < q) {
+      const uint4 v = p[u];
+      rec[4 * u] = v.x;
+      rec[4 * u + 1] = v.y;
+      rec[4 * u + 2] = v.z;
+      rec[4 * u + 3] = v.w;
     }
-    if (w + k > 7) {
-      acc = carry(acc);
-      w = 1;
-    }
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-      if (j < k) {
-#pragma unroll
-        for (int i = 0; i < NL; i++) acc.l[i] = SUB ? acc.l[i] - x[j].l[i] : acc.l[i] + x[j].l[i];
-      }
-    }
-    w += k;
   }
 }
-__device__ __forceinline__ Fp gather(int sched_w, int at, int npos, int nneg, const Regions& R) {
-  Fp acc = fp_zero();
-  int w = 0;
-  gather_sign<false>(acc, w, sched_w, at, npos, R);
-  gather_sign<true>(acc, w, sched_w, at + npos, nneg, R);
-  return acc;
+
+// acc += sum of the n terms at halfwords FIRST.. of the record (n uniform, n <= MAXN).
+// Terms are processed four at a time, their LDS loads issued before the arithmetic;
+// each term is a slot times a signed coefficient, accumulated in 64 bits per limb
+// (one v_mad_i64_i32 per limb).  Zero-coefficient terms are padding.
+template <int FIRST, int MAXN>
+__device__ __forceinline__ void accumulate(i64 (&acc)[NL], const u32 (&rec)[REC_WORDS], int n, const Regions& R) {
+#pragma unroll
+  for (int c = 0; c < MAXN; c += 4) {
+    if (c < n) {
+      Fp x[4];
+      i32 k[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (c + j < MAXN && c + j < n) {
+          const u32 t = rec_hw(rec, FIRST + c + j);
+          x[j] = lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
+          k[j] = (i32)(int8_t)(t >> 8);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (c + j < MAXN && c + j < n) {
+#pragma unroll
+          for (int i = 0; i < NL; i++) acc[i] += (i64)x[j].l[i] * k[j];
+        }
+      }
+    }
+  }
 }
 
-// One schedule for the item of this lane's group.  Outputs go to D[0..NOUT); D may
-// alias A (every schedule used in place reads no other input component in its outputs).
-// Not inlined: one copy of the multiplier serves every call site.
-__device__ __noinline__ void run_op_impl(int G, int op, int sub, bool active, int A, int B, int D, int T, int K,
-                                         int sched_w) {
+// 64-bit limb sums -> limbs 0..12 in [0, 2^28), the top limb signed (value unchanged)
+__device__ __forceinline__ Fp carry64(const i64 (&acc)[NL]) {
+  Fp r;
+  i64 c = 0;
+#pragma unroll
+  for (int i = 0; i < NL - 1; i++) {
+    const i64 v = acc[i] + c;
+    r.l[i] = (i32)v & MASK28;
+    c = v >> 28;
+  }
+  r.l[NL - 1] = (i32)(acc[NL - 1] + c);
+  return r;
+}
+
+template <class P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (P*)(((uint64_t)hi << 32) | lo);
+}
+
+// One operation for the item of this lane's group, as rounds of the per-G tables:
+// product rounds (each lane multiplies at most one pair of operand sums into T), a
+// barrier, output rounds (each lane forms at most one output in D), a barrier.  Every
+// lane of the wavefront runs the same instruction stream whatever its record holds.
+// D may alias A (every op used in place reads no other input component in its
+// outputs).  Not inlined: one copy of the multiplier serves every call site.
+__device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* rounds, const uint4* __restrict__ desc,
+                                         int op, int sub, bool active, int A, int B, int D, int T, int K) {
+  // the table pointers, the op and the round shapes are the same on every lane: make
+  // that visible to the compiler so that they live in scalar registers, the tables are
+  // read with scalar loads and every branch on them is a scalar branch
+  ops = uniform_ptr(ops);
+  rounds = uniform_ptr(rounds);
+  desc = uniform_ptr(desc);
+  op = __builtin_amdgcn_readfirstlane(op);
+  K = __builtin_amdgcn_readfirstlane(K);
   const Regions R{A, B, T, K};
-  const int np = sched::NPROD[op], no = sched::NOUT[op], so = sched::START_OFF[op];
-  if (active) {
-    for (int j = sub; j < np; j += G) {
-      const int d = start_of(sched_w, so + j);
-      const int dst = code_byte(sched_w, d), ap = code_byte(sched_w, d + 1), an = code_byte(sched_w, d + 2);
-      const int bp = code_byte(sched_w, d + 3), bn = code_byte(sched_w, d + 4);
-      const Fp a = carry(gather(sched_w, d + 5, ap, an, R));
-      const Fp b = carry(gather(sched_w, d + 5 + ap + an, bp, bn, R));
-      lds_st(T + dst * SLOT, mul(a, b));
+  const int first = ops[op].first, np = ops[op].np, nr = np + ops[op].no;
+  u32 cur[REC_WORDS], nxt[REC_WORDS];
+  if (nr > 0) load_record(cur, desc, rounds[first].desc, rounds[first].q, sub);
+  for (int r = 0; r < nr; r++) {
+    const int na = rounds[first + r].na, nb = rounds[first + r].nb;
+    if (r + 1 < nr) load_record(nxt, desc, rounds[first + r + 1].desc, rounds[first + r + 1].q, sub);
+    const u32 dst = rec_hw(cur, 0);
+    i64 acc[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) acc[i] = 0;
+    if (r < np) {
+      accumulate<1, 8>(acc, cur, na, R);
+      const Fp a = carry64(acc);
+#pragma unroll
+      for (int i = 0; i < NL; i++) acc[i] = 0;
+      accumulate<sched::B_AT, 8>(acc, cur, nb, R);
+      const Fp b = carry64(acc);
+      const Fp p = mul(a, b);
+      if (active && dst != sched::IDLE) lds_st(T + (int)dst * SLOT, p);
+      if (r + 1 == np) __syncthreads();
+    } else {
+      accumulate<1, 2 * REC_WORDS - 1>(acc, cur, na, R);
+      const Fp o = fold_top(carry64(acc));
+      if (active && dst != sched::IDLE) lds_st(D + (int)dst * SLOT, o);
     }
-  }
-  __syncthreads();
-  if (active) {
-    for (int o = sub; o < no; o += G) {
-      const int d = start_of(sched_w, so + np + o);
-      const int dst = code_byte(sched_w, d), np_ = code_byte(sched_w, d + 1), nn = code_byte(sched_w, d + 2);
-      lds_st(D + dst * SLOT, norm(gather(sched_w, d + 3, np_, nn, R)));
+    if (r + 1 < nr) {
+#pragma unroll
+      for (int w = 0; w < REC_WORDS; w++) cur[w] = nxt[w];
     }
   }
   __syncthreads();
 }
-// the schedule copy sits at the end of the dynamic LDS; each kernel passes its offset
-// through K: the constants region is followed by the schedule (see *_lds_bytes)
 template <int G>
 __device__ __forceinline__ void run_op(int op, int sub, bool active, int A, int B, int D, int T, int K) {
-  run_op_impl(G, op, sub, active, A, B, D, T, K, K + sched::NUM_CONSTS * SLOT);
+  run_op_impl(sched::Tab<G>::ops(), sched::Tab<G>::rounds(), sched::Tab<G>::desc(), op, sub, active, A, B, D, T, K);
 }
 
 __device__ __forceinline__ void load_consts(int K) {
@@ -160,7 +205,7 @@ constexpr int ML_SLOTS = 79;
 
 template <int G>
 constexpr size_t miller_lds_bytes() {
-  return ((size_t)((64 / G) * ML_SLOTS + sched::NUM_CONSTS) * SLOT + SCHED_WORDS) * 4;
+  return (size_t)((64 / G) * ML_SLOTS + sched::NUM_CONSTS) * SLOT * 4;
 }
 
 template <int G>
@@ -271,7 +316,6 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * ML_SLOTS * SLOT;
   load_consts(K);
-  load_schedule(K + sched::NUM_CONSTS * SLOT);
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
     if (tile >= ntiles) break;
@@ -286,7 +330,7 @@ constexpr int FE_SLOTS = 126;
 
 template <int G>
 constexpr size_t final_exp_lds_bytes() {
-  return ((size_t)((64 / G) * FE_SLOTS + sched::NUM_CONSTS) * SLOT + SCHED_WORDS + (64 / G) * 12) * 4;
+  return ((size_t)((64 / G) * FE_SLOTS + sched::NUM_CONSTS) * SLOT + (64 / G) * 12) * 4;
 }
 
 template <int G>
@@ -313,7 +357,7 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
   const int R0 = base, R1 = base + 12 * SLOT, R2 = base + 24 * SLOT, R3 = base + 36 * SLOT;
   const int R4 = base + 48 * SLOT, R5 = base + 60 * SLOT, T = base + 72 * SLOT;
   const int K = IPW * FE_SLOTS * SLOT;
-  const int FLAGS = K + sched::NUM_CONSTS * SLOT + SCHED_WORDS + (in_wg ? lid : 0) * 12;
+  const int FLAGS = K + sched::NUM_CONSTS * SLOT + (in_wg ? lid : 0) * 12;
   int32_t st = DEC_OK;
   if (valid && status) st = status[it];
   if (valid && sub_status && st == DEC_OK) st = sub_status[it];
@@ -421,7 +465,6 @@ __global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fi
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * FE_SLOTS * SLOT;
   load_consts(K);
-  load_schedule(K + sched::NUM_CONSTS * SLOT);
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
     if (tile >= ntiles) break;

@@ -1,0 +1,130 @@
+"""The per-G round tables that drive the lanes-per-item kernels (icp-gpu_amd/csrc/
+gen_schedules.py -> schedules_gen.hpp), executed here with Python big integers exactly as
+the kernel's round interpreter reads them: records per lane, uniform term counts, zero
+padding, regions A/B/T/K, signed coefficients.  Every op at every lane count must give
+the values of its symbolic definition, and the Fp12 product and square must equal the
+independent schoolbook model (tests/pyfield.py), so a mistake in round assignment, record
+layout or term encoding fails here, on the CPU."""
+import os
+import random
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc"))
+import gen_schedules as GSCH  # noqa: E402
+import pyfield as PF  # noqa: E402
+
+P = GSCH.P
+A, B, T, K = GSCH.A, GSCH.B, GSCH.T, GSCH.K
+
+
+def decode(term):
+    coef = (term >> 8) & 0xFF
+    if coef >= 128:
+        coef -= 256
+    return (term >> 6) & 3, term & 63, coef
+
+
+def run_op(rounds, a, b):
+    """execute one op's rounds (as the kernel does) on input values a, b (lists of ints)"""
+    prounds, orounds = rounds
+    regions = {A: a, B: b, K: [v % P for v in GSCH.CONSTS.vals], T: {}}
+
+    def val(term):
+        reg, idx, coef = decode(term)
+        if coef == 0:
+            return 0
+        return coef * regions[reg][idx]
+
+    for na, nb, recs in prounds:
+        for rec in recs:
+            assert len(rec) == 8 * GSCH.PROD_Q
+            if rec[0] == GSCH.IDLE:
+                continue
+            # the kernel reads exactly na and nb terms at fixed places
+            assert all(decode(t)[2] == 0 for t in rec[1 + na:GSCH.B_AT] + rec[GSCH.B_AT + nb:])
+            x = sum(val(t) for t in rec[1:1 + na])
+            y = sum(val(t) for t in rec[GSCH.B_AT:GSCH.B_AT + nb])
+            assert rec[0] not in regions[T]
+            regions[T][rec[0]] = x * y % P
+    out = {}
+    for n, _, recs in orounds:
+        for rec in recs:
+            if rec[0] == GSCH.IDLE:
+                continue
+            assert all(decode(t)[2] == 0 for t in rec[1 + n:])
+            assert rec[0] not in out
+            out[rec[0]] = sum(val(t) for t in rec[1:1 + n]) % P
+    return [out[i] for i in range(len(out))]
+
+
+def symbolic(prods, outs, a, b):
+    vals = {A: a, B: b, K: [v % P for v in GSCH.CONSTS.vals]}
+
+    def ev(lin, tv):
+        s = 0
+        for (reg, idx), c in lin.t.items():
+            s += c * (tv[idx] if reg == T else vals[reg][idx])
+        return s % P
+
+    tv = [ev(x, None) * ev(y, None) % P for x, y in prods]
+    return [ev(o, tv) for o in outs]
+
+
+@pytest.fixture(scope="module")
+def progs():
+    return GSCH.op_programs()
+
+
+@pytest.mark.parametrize("G", GSCH.GS)
+def test_round_tables_execute_every_op(progs, G):
+    rnd = random.Random(G)
+    ops = GSCH.build_rounds(G, progs)
+    for (name, prods, outs, _), (name2, prounds, orounds) in zip(progs, ops):
+        assert name == name2
+        # every lane record of a round has the round's shape; idle lanes only at the end
+        for na, nb, recs in prounds + orounds:
+            assert len(recs) == G
+            busy = [r[0] != GSCH.IDLE for r in recs]
+            assert busy == sorted(busy, reverse=True)
+        assert sum(1 for _, _, recs in prounds for r in recs if r[0] != GSCH.IDLE) == len(prods)
+        assert sum(1 for _, _, recs in orounds for r in recs if r[0] != GSCH.IDLE) == len(outs)
+        for _ in range(3):
+            a = [rnd.randrange(P) for _ in range(12)]
+            b = [rnd.randrange(P) for _ in range(12)]
+            assert run_op((prounds, orounds), a, b) == symbolic(prods, outs, a, b), name
+
+
+def tower(vals):
+    return [(vals[2 * k], vals[2 * k + 1]) for k in range(6)]
+
+
+def flat(t):
+    return [x for pair in t for x in pair]
+
+
+@pytest.mark.parametrize("G", [13, 16, 18])
+def test_fp12_mul_and_square_against_schoolbook(progs, G):
+    rnd = random.Random(100 + G)
+    ops = {name: (p, o) for name, p, o in GSCH.build_rounds(G, progs)}
+    mul = ops["OP_MUL"]
+    sqr = ops["OP_SQR"]
+    for _ in range(4):
+        a = [rnd.randrange(P) for _ in range(12)]
+        b = [rnd.randrange(P) for _ in range(12)]
+        want = PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(a)), PF.f12_from_tower(tower(b))))
+        assert run_op(mul, a, b) == flat(want)
+        want2 = PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(a)), PF.f12_from_tower(tower(a))))
+        assert run_op(sqr, a, [0] * 12) == flat(want2)
+
+
+def test_generated_header_is_current(progs):
+    """schedules_gen.hpp is what the generator writes now (the build does not regenerate it)"""
+    path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
+    text = open(path).read()
+    for G in GSCH.GS:
+        tables, nbytes = GSCH.emit_tables(G, GSCH.build_rounds(G, progs))
+        for line in tables:
+            assert line in text, "schedules_gen.hpp is stale: run gen_schedules.py (G=%d)" % G
