@@ -240,7 +240,7 @@ def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune):
                                       cfg.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p)):
             raise RuntimeError("autotune: " + blsgpu.last_error())
         tuned = {"lanes_miller": int(cfg[0]), "lanes_final_exp": int(cfg[1]), "wg_per_cu_miller": int(cfg[2]),
-                 "wg_per_cu_final_exp": int(cfg[3]), "miller_ms": float(ms[0]), "final_exp_ms": float(ms[1]),
+                 "wg_per_cu_final_exp": int(cfg[3]), "sequence_ms": float(ms[0]), "final_exp_ms": float(ms[1]),
                  "tuning_s": round(time.perf_counter() - t_t, 2)}
         log("autotuned for %d items: %s" % (n, tuned))
 
@@ -381,7 +381,7 @@ def main():
         if lib.blsgpu_autotune_device(d_sig.data_ptr(), d_nhm.data_ptr(), d_key.data_ptr(), n, 1, sp,
                                       cfg.ctypes.data_as(vp), ms.ctypes.data_as(vp)):
             raise RuntimeError("autotune: " + blsgpu.last_error())
-        tuning = {"miller_ms": float(ms[0]), "final_exp_ms": float(ms[1]),
+        tuning = {"sequence_ms": float(ms[0]), "final_exp_ms": float(ms[1]),
                   "tuning_s": round(time.perf_counter() - t_t, 2)}
         args.engine = "1:%d:%d" % (cfg[0], cfg[1])
     mode, gml, gfe = (int(x) for x in args.engine.split(":"))

@@ -413,8 +413,7 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
 int blsgpu_set_engine(int mode, int g_miller, int g_final_exp) {
   std::lock_guard<std::mutex> lk(g.mu);
   g_err.clear();
-  const bool ok_ml = g_miller == 8 || g_miller == 12 || g_miller == 13 || g_miller == 16 || g_miller == 32;
-  const bool ok_fe = g_final_exp == 12 || g_final_exp == 16 || g_final_exp == 18 || g_final_exp == 32;
+  const bool ok_ml = bls::grp_lanes_supported(0, g_miller), ok_fe = bls::grp_lanes_supported(1, g_final_exp);
   if ((mode != 0 && mode != 1) || (mode == 1 && (!ok_ml || !ok_fe))) {
     g_err = "unsupported engine configuration";
     return -1;

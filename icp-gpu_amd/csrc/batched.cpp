@@ -61,8 +61,8 @@ BatchedCfg cfg_for_locked(int M) {
 
 namespace {
 
-bool valid_g_ml(int G) { return G == 8 || G == 12 || G == 13 || G == 16 || G == 32; }
-bool valid_g_fe(int G) { return G == 12 || G == 16 || G == 18 || G == 32; }
+bool valid_g_ml(int G) { return bls::grp_lanes_supported(0, G); }
+bool valid_g_fe(int G) { return bls::grp_lanes_supported(1, G); }
 
 int ensure_batched_ws(int M) {
   const size_t m = (size_t)M;
@@ -337,12 +337,12 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
   bls::EngineArgs a =
       batched_args(d_sigs48, d_neg_hms, d_key_idx, g.b_res.as<int32_t>(), nullptr, M, c);
   CHK(bls::engine_prepare(a, g.s));
-  // times one phase (0 Miller, 1 final exponentiation) of configuration a; best of 2
+  // times one phase (1 final exponentiation, 2 the whole sequence) of configuration a; best of 2
   auto time_phase = [&](int which, double* ms) -> int {
     float best = 1e30f;
     for (int r = 0; r < 2; r++) {
       HIPCHK(hipEventRecord(e0, g.s));
-      CHK(which == 0 ? bls::engine_miller(a, g.s) : bls::engine_final_exp(a, g.s));
+      CHK(which == 1 ? bls::engine_final_exp(a, g.s) : bls::engine_compressed(a, g.s));
       HIPCHK(hipEventRecord(e1, g.s));
       HIPCHK(hipEventSynchronize(e1));
       float t = 0;
@@ -352,30 +352,14 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
     *ms = best;
     return 0;
   };
-  const int gml[] = {8, 12, 13, 16, 32}, gfe[] = {12, 16, 18, 32};
+  const int gml[] = {8, 12, 13, 16, 20, 32}, gfe[] = {9, 12, 16, 18, 27, 32};
+  // 1. the final exponentiation alone, on the Miller output of the current configuration
+  CHK(bls::engine_miller(a, g.s));
   double best_ml = 1e30, best_fe = 1e30;
-  for (int G : gml) {
-    const int occ = form == 0 ? bls::occupancy_grp(0, G) : -1;
-    for (int w = -1; w <= occ; w++) {
-      if (w == 0) continue;  // the same grid as w == occ
-      a.g_ml = G;
-      a.wpc_ml = w;
-      double t = 0;
-      CHK(time_phase(0, &t));
-      if (t < best_ml) {
-        best_ml = t;
-        c.g_ml = G;
-        c.wpc_ml = w;
-      }
-    }
-  }
-  a.g_ml = c.g_ml;
-  a.wpc_ml = c.wpc_ml;
-  CHK(bls::engine_miller(a, g.s));  // the final exponentiation candidates read this output
   for (int G : gfe) {
     const int occ = form == 0 ? bls::occupancy_grp(1, G) : -1;
     for (int w = -1; w <= occ; w++) {
-      if (w == 0) continue;
+      if (w == 0) continue;  // the same grid as w == occ
       a.g_fe = G;
       a.wpc_fe = w;
       double t = 0;
@@ -384,6 +368,27 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
         best_fe = t;
         c.g_fe = G;
         c.wpc_fe = w;
+      }
+    }
+  }
+  a.g_fe = c.g_fe;
+  a.wpc_fe = c.wpc_fe;
+  // 2. the Miller candidates inside the whole sequence (decode, then the Miller loop with
+  // the subgroup test running concurrently, then the final exponentiation): the
+  // concurrent test competes for the SIMDs, which favours configurations that leave
+  // some free, so the Miller loop is not timed alone
+  for (int G : gml) {
+    const int occ = form == 0 ? bls::occupancy_grp(0, G) : -1;
+    for (int w = -1; w <= occ; w++) {
+      if (w == 0) continue;
+      a.g_ml = G;
+      a.wpc_ml = w;
+      double t = 0;
+      CHK(time_phase(2, &t));
+      if (t < best_ml) {
+        best_ml = t;
+        c.g_ml = G;
+        c.wpc_ml = w;
       }
     }
   }

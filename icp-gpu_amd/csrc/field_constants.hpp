@@ -38,4 +38,7 @@ constexpr uint32_t HALF_P32[12] = {0xffffd555u, 0xdcff7fffu, 0x58a9ffffu, 0x0f55
 constexpr uint32_t P32[12] = {0xffffaaabu, 0xb9feffffu, 0xb153ffffu, 0x1eabfffeu, 0xf6b0f624u, 0x6730d2a0u, 0xf38512bfu, 0x64774b84u, 0x434bacd7u, 0x4b1ba7b6u, 0x397fe69au, 0x1a0111eau};
 constexpr uint32_t R_ORDER32[8] = {0x00000001u, 0xffffffffu, 0xfffe5bfeu, 0x53bda402u, 0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
 constexpr uint64_t BLS_X = 0xd201000000010000ull;  // x = -BLS_X
+// (p+1)/4 as a 4-bit sliding window: {squarings, odd digit} (see gen_constants.py)
+constexpr int SQRT_WIN_LEN = 79;
+constexpr uint8_t SQRT_WIN[79][2] = {{4, 13}, {9, 1}, {4, 1}, {7, 15}, {4, 5}, {6, 7}, {6, 11}, {4, 15}, {4, 15}, {6, 13}, {6, 13}, {6, 9}, {3, 3}, {7, 13}, {4, 13}, {6, 15}, {5, 13}, {4, 9}, {8, 13}, {6, 11}, {3, 5}, {3, 3}, {6, 13}, {4, 7}, {3, 3}, {3, 1}, {6, 7}, {4, 7}, {5, 9}, {4, 7}, {8, 9}, {3, 7}, {5, 7}, {7, 5}, {7, 9}, {5, 11}, {4, 15}, {3, 3}, {5, 7}, {4, 3}, {8, 13}, {5, 5}, {2, 1}, {9, 15}, {5, 13}, {3, 3}, {8, 15}, {3, 3}, {7, 9}, {9, 15}, {4, 5}, {5, 11}, {4, 15}, {4, 15}, {4, 15}, {3, 7}, {5, 11}, {6, 5}, {5, 9}, {4, 15}, {4, 15}, {4, 15}, {4, 15}, {4, 13}, {2, 3}, {6, 15}, {4, 15}, {5, 15}, {4, 15}, {4, 15}, {4, 15}, {4, 15}, {4, 15}, {4, 15}, {4, 15}, {3, 5}, {4, 5}, {4, 5}, {3, 3}};
 }  // namespace bls

@@ -142,6 +142,31 @@ def main():
     emit("constexpr uint32_t P32[12] = {%s};" % words(P, 12))
     emit("constexpr uint32_t R_ORDER32[8] = {%s};" % words(R_ORDER, 8))
     emit("constexpr uint64_t BLS_X = 0x%016xull;  // x = -BLS_X" % BLS_X)
+    # (p+1)/4 as a left-to-right sliding window of 4 bits: the first entry's odd digit
+    # starts the result; every later entry squares `sq` times, then multiplies by
+    # a^digit (digit odd, < 16); a final entry with digit 0 only squares
+    e = (P + 1) // 4
+    bits = bin(e)[2:]
+    win, i, pend = [], 0, 0
+    while i < len(bits):
+        if bits[i] == "0":
+            pend += 1
+            i += 1
+            continue
+        j = min(i + 4, len(bits))
+        while bits[j - 1] == "0":
+            j -= 1
+        win.append((pend + (j - i), int(bits[i:j], 2)))
+        pend, i = 0, j
+    if pend:
+        win.append((pend, 0))
+    chk = 0
+    for k, (sq, d) in enumerate(win):
+        chk = d if k == 0 else (chk << sq) + d
+    assert chk == e and all(d % 2 == 1 for _, d in win if d) and win[0][1] > 0
+    emit("// (p+1)/4 as a 4-bit sliding window: {squarings, odd digit} (see gen_constants.py)")
+    emit("constexpr int SQRT_WIN_LEN = %d;" % len(win))
+    emit("constexpr uint8_t SQRT_WIN[%d][2] = {%s};" % (len(win), ", ".join("{%d, %d}" % w for w in win)))
     emit("}  // namespace bls")
     with open(os.path.join(HERE, "field_constants.hpp"), "w") as f:
         f.write("\n".join(out) + "\n")

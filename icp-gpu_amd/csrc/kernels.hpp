@@ -23,8 +23,8 @@ int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
                      int block, hipStream_t s);
 
-// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13,
-// 16, 32; final exponentiation: 12, 16, 18, 32).  Return -1 for an unsupported G or a launch error.
+// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13, 16,
+// 20, 32; final exponentiation: 9, 12, 16, 18, 27, 32).  Return -1 for an unsupported G or a launch error.
 // queue == nullptr: one workgroup per tile of 64/G items.  Otherwise a persistent grid
 // (at most wg_per_cu workgroups per CU, 0 = the occupancy limit) pops tiles from the
 // device counter *queue, which must be zero at launch.
@@ -33,6 +33,8 @@ int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32
                       int* queue, int wg_per_cu, hipStream_t s);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
                          uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s);
+// whether a lanes-per-item kernel (which: 0 Miller, 1 final exp) exists for G lanes
+bool grp_lanes_supported(int which, int G);
 // resident workgroups per CU of a grouped kernel (which: 0 Miller, 1 final exp), -1 if G is unsupported
 int occupancy_grp(int which, int G);
 // p[0..n) = 0 by a kernel (safe inside captured graphs)
@@ -43,5 +45,8 @@ int launch_zero_words(int* p, int n, hipStream_t s);
 int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s);
 int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s);
 int launch_merge_status(int32_t* status, const int32_t* sub_status, int M, hipStream_t s);
+// result := sub_status (and GT := 0) for items that decoded but failed the subgroup test
+int launch_apply_subgroup(int32_t* result, uint64_t* gt, const int32_t* status, const int32_t* sub_status, int M,
+                          hipStream_t s);
 
 }  // namespace bls

@@ -100,6 +100,25 @@ __global__ void k_merge_status(i32* __restrict__ status, const i32* __restrict__
   if (status[i] == DEC_OK && sub_status[i] != DEC_OK) status[i] = sub_status[i];
 }
 
+// After a final exponentiation that ran before the subgroup test finished: items that
+// decoded but are not in G1 get the subgroup status as their result, and a zero GT
+// (the value every rejected item exports).
+__global__ void k_apply_subgroup(int32_t* __restrict__ result, uint64_t* __restrict__ gt, const i32* __restrict__ status,
+                                 const i32* __restrict__ sub_status, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  if (status[i] != DEC_OK || sub_status[i] == DEC_OK) return;
+  result[i] = sub_status[i];
+  if (gt) {
+    for (int j = 0; j < 72; j++) gt[(size_t)i * 72 + j] = 0;
+  }
+}
+
+int launch_apply_subgroup(int32_t* result, uint64_t* gt, const int32_t* status, const int32_t* sub_status, int M,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_subgroup, dim3((M + 255) / 256), dim3(256), 0, s, result, gt, status, sub_status, M);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s) {
   hipLaunchKernelGGL(k_g1_decode, dim3((M + 63) / 64), dim3(64), 0, s, in, pts, status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;

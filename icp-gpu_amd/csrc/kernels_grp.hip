@@ -315,6 +315,9 @@ __global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, 
   constexpr int IPW = 64 / G;
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * ML_SLOTS * SLOT;
+  // raised wave priority: a concurrent subgroup-test wave on the same SIMD issues in the
+  // cycles this wave leaves idle instead of halving its issue rate
+  __builtin_amdgcn_s_setprio(2);
   load_consts(K);
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
@@ -464,6 +467,7 @@ __global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fi
   constexpr int IPW = 64 / G;
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * FE_SLOTS * SLOT;
+  __builtin_amdgcn_s_setprio(2);
   load_consts(K);
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
@@ -531,7 +535,7 @@ int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32
   switch (G) {
 #define ML_CASE(g) \
   case g: return launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, s);
-    ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(32)
+    ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(20) ML_CASE(32)
 #undef ML_CASE
     default: return -1;
   }
@@ -540,10 +544,15 @@ int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const
                          uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
   switch (G) {
 #define FE_CASE(g) case g: return launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, s);
-    FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(32)
+    FE_CASE(9) FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(27) FE_CASE(32)
 #undef FE_CASE
     default: return -1;
   }
+}
+
+bool grp_lanes_supported(int which, int G) {
+  if (which == 0) return G == 8 || G == 12 || G == 13 || G == 16 || G == 20 || G == 32;
+  return G == 9 || G == 12 || G == 16 || G == 18 || G == 27 || G == 32;
 }
 
 int occupancy_grp(int which, int G) {
@@ -553,14 +562,17 @@ int occupancy_grp(int which, int G) {
       case 12: return ml_occ<12>();
       case 13: return ml_occ<13>();
       case 16: return ml_occ<16>();
+      case 20: return ml_occ<20>();
       case 32: return ml_occ<32>();
       default: return -1;
     }
   }
   switch (G) {
+    case 9: return fe_occ<9>();
     case 12: return fe_occ<12>();
     case 16: return fe_occ<16>();
     case 18: return fe_occ<18>();
+    case 27: return fe_occ<27>();
     case 32: return fe_occ<32>();
     default: return -1;
   }

@@ -35,7 +35,11 @@ static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
   return a.ev ? (hipEventRecord(a.ev[k], s) == hipSuccess ? 0 : -1) : 0;
 }
 
-// decode -> fork: [subgroup test on s2] || [Miller loop on s] -> join -> final exp
+// decode -> fork: [subgroup test on s2] || [Miller loop -> final exp on s] -> join ->
+// subgroup verdicts applied.  The subgroup test does not feed the pairing, so in the
+// lanes-per-item mode it overlaps both pairing kernels (which run at a raised wave
+// priority, so the test takes the issue slots they leave); the thread-per-item mode
+// joins before its final exponentiation, which reads the merged status.
 int engine_compressed(const EngineArgs& a, hipStream_t s) {
   if (mark(a, 0, s)) return -1;
   if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 2, s)) return -1;
@@ -50,8 +54,13 @@ int engine_compressed(const EngineArgs& a, hipStream_t s) {
     return -1;
   }
   if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s)) return -1;
-  if (fork && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
-  if (mark(a, 3, s) || final_exp(a, a.status, a.sub_status, s)) return -1;
+  const bool late = a.mode == 1;
+  if (fork && !late && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
+  if (mark(a, 3, s) || final_exp(a, a.status, late ? nullptr : a.sub_status, s)) return -1;
+  if (late) {
+    if (fork && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
+    if (launch_apply_subgroup(a.results, a.gt, a.status, a.sub_status, a.M, s)) return -1;
+  }
   return mark(a, 4, s);
 }
 

@@ -127,8 +127,8 @@ int blsgpu_verify_batches_device(const uint8_t* d_sigs48, const uint64_t* d_neg_
 int blsgpu_verify_batches(const uint8_t* const* sigs48, const uint64_t* const* neg_hms,
                           const int32_t* const* key_idx, const int32_t* sizes, int nbatches,
                           int32_t* const* results, uint64_t* const* gt_out);
-/* Batched-form configuration: lanes per item of the Miller loop (8, 12, 13, 16, 32) and of
- * the final exponentiation (12, 16, 18, 32), workgroups per CU of each (0 = occupancy
+/* Batched-form configuration: lanes per item of the Miller loop (8, 12, 13, 16, 20, 32) and of
+ * the final exponentiation (9, 12, 16, 18, 27, 32), workgroups per CU of each (0 = occupancy
  * limit, -1 = one workgroup per tile, no queue), and graph capture on/off.  Clears the
  * autotuned table. */
 int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int wpc_final_exp, int use_graph);
@@ -137,20 +137,24 @@ int blsgpu_get_batched_engine(int M, int cfg[5]);
 /* out[0] graphs instantiated, out[1] graph launches */
 int blsgpu_batched_stats(int out[2]);
 /* Autotuner on a device-resident job of M items, timed with HIP events (best of 2).
- * form 0 (batched): times every lanes-per-item and workgroups-per-CU candidate of the
- *   Miller loop, then of the final exponentiation, and records the fastest pair for M's
- *   size class (ceil(log2 M)); later batched calls and serving steps use the entry of
- *   the nearest class.
- * form 1 (plain): times the lanes-per-item candidates with one workgroup per tile and
- *   makes the fastest pair the plain form's engine (as blsgpu_set_engine(1, ...)).
+ * It times every final-exponentiation candidate alone, then every Miller-loop candidate
+ * inside the whole kernel sequence with that final exponentiation (the subgroup test
+ * runs concurrently with the pairing kernels, so their best configuration depends on
+ * it).
+ * form 0 (batched): candidates are lanes per item x workgroups per CU; the fastest pair
+ *   is recorded for M's size class (ceil(log2 M)); later batched calls and serving steps
+ *   use the entry of the nearest class.
+ * form 1 (plain): lanes-per-item candidates with one workgroup per tile; the fastest
+ *   pair becomes the plain form's engine (as blsgpu_set_engine(1, ...)).
  * cfg_out (nullable) = {g_miller, g_final_exp, wpc_miller, wpc_final_exp}; ms_out
- * (nullable) = the two best kernel times in ms.  Synchronous. */
+ * (nullable) = {best whole-sequence time, best final-exponentiation time} in ms.
+ * Synchronous. */
 int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx, int M,
                            int form, void* stream, int cfg_out[4], double ms_out[2]);
 
 /* Engine selection: mode 0 = one thread per verification (thread-per-item kernels),
  * mode 1 = a group of G lanes of one wavefront per verification (g_miller in
- * {8, 12, 13, 16, 32} for the Miller loop, g_final_exp in {12, 16, 18, 32} for the final
+ * {8, 12, 13, 16, 20, 32} for the Miller loop, g_final_exp in {9, 12, 16, 18, 27, 32} for the final
  * exponentiation).  The
  * default is mode 1 with 12/16. */
 int blsgpu_set_engine(int mode, int g_miller, int g_final_exp);
