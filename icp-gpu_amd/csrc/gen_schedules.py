@@ -419,6 +419,64 @@ def op_scale2():
     return R, out
 
 
+# ---- merged line steps of the Miller loop ----
+# Region B of these ops is the item's line block: L1 (B 0..5), L2 (B 6..11) = next lines
+# (c0, c1, c2) of the two pairs, P (B 12..15) = (P1.x, P1.y, P2.x, P2.y), S (B 16..27) =
+# the scaled lines (c2, c1 P.x, c0 P.y) of pair 1 then pair 2, which the line product
+# overwrites with LL (B 16..25).  A line is the sparse element x0 + x1 v + y1 v w.
+ZERO2 = F2(Lin(), Lin())
+
+
+def line_product(R, l1, l2):
+    """(x0 + x1 v + y1 v w)(u0 + u1 v + z1 v w) with w^2 = v, v^3 = 1 + u: six Fp2 products
+    (three of them Karatsuba sums); the w^0 part is full, the w^1 part has no 1-term"""
+    x0, x1, y1 = l1
+    u0, u1, z1 = l2
+    t00, t11, tyy = f2mul(R, x0, u0), f2mul(R, x1, u1), f2mul(R, y1, z1)
+    s01 = f2mul(R, x0 + x1, u0 + u1) - t00 - t11
+    s0y = f2mul(R, x0 + y1, u0 + z1) - t00 - tyy
+    s1y = f2mul(R, x1 + y1, u1 + z1) - t11 - tyy
+    return F12(F6(t00 + tyy.nr(), s01, t11), F6(ZERO2, s0y, s1y))
+
+
+def op_ll():
+    # LL = line1 * line2 from the scaled lines S (B 16..27); outputs (c0.c0, c0.c1, c0.c2,
+    # c1.c1, c1.c2), 10 Fp written over S
+    R = Rec()
+    l1 = [f2(B, 8), f2(B, 9), f2(B, 10)]
+    l2 = [f2(B, 11), f2(B, 12), f2(B, 13)]
+    ll = line_product(R, l1, l2)
+    return R, outs_of(ll.c0.c[0]) + outs_of(ll.c0.c[1]) + outs_of(ll.c0.c[2]) + outs_of(ll.c1.c[1]) + \
+        outs_of(ll.c1.c[2])
+
+
+def ll_operand():
+    return F12(F6(f2(B, 8), f2(B, 9), f2(B, 10)), F6(ZERO2, f2(B, 11), f2(B, 12)))
+
+
+def scaled_lines(R):
+    """(c2, c1 P.x, c0 P.y) of both pairs from L1, L2 and P (api.cu:37)"""
+    out = []
+    for pair in range(2):
+        c0, c1, c2 = f2(B, 3 * pair), f2(B, 3 * pair + 1), f2(B, 3 * pair + 2)
+        x, y = fp(B, 12 + 2 * pair), fp(B, 13 + 2 * pair)
+        out += outs_of(c2) + outs_of(f2mul_fp(R, c1, x)) + outs_of(f2mul_fp(R, c0, y))
+    return out
+
+
+def op_fl_scale():
+    # f (A) times LL, and the next step's scaled lines: outputs f at 0..11 and the
+    # scaled lines at 28..39 (the item's S slots, relative to f)
+    R = Rec()
+    outs = outs_of(f12mul(R, f12(A), ll_operand()))
+    return R, outs + [None] * 16 + scaled_lines(R)
+
+
+def op_fl():
+    R = Rec()
+    return R, outs_of(f12mul(R, f12(A), ll_operand()))
+
+
 OPS = [
     ("OP_MUL", op_mul),
     ("OP_SQR", op_sqr),
@@ -439,6 +497,9 @@ OPS = [
     ("OP_INV6", op_inv6),
     ("OP_INV12", op_inv12),
     ("OP_SCALE2", op_scale2),
+    ("OP_LL", op_ll),
+    ("OP_FL_SCALE", op_fl_scale),
+    ("OP_FL", op_fl),
 ]
 
 
@@ -489,11 +550,11 @@ def op_programs():
             assert len(a.t) <= MAX_OPERAND_TERMS and len(b.t) <= MAX_OPERAND_TERMS, name
             assert weight(a) <= 16 and weight(b) <= 16, name
         for o in outs:
-            assert weight(o) < 4096 and len(o.t) <= 8 * MAX_OUT_Q - 1, name
+            assert o is None or (weight(o) < 4096 and len(o.t) <= 8 * MAX_OUT_Q - 1), name
         # hazard check: an output that reads input A index k other than its own index
         # would see a value overwritten in place; the kernels write such outputs to a
         # distinct register.
-        reads_other = any(any(r == A and idx != i for (r, idx) in o.t) for i, o in enumerate(outs))
+        reads_other = any(any(r == A and idx != i for (r, idx) in o.t) for i, o in enumerate(outs) if o is not None)
         progs.append((name, list(R.products), outs, reads_other))
     return progs
 
@@ -522,7 +583,7 @@ def build_rounds(G, progs):
                     rec[B_AT:B_AT + len(tb)] = tb
                 recs.append(rec)
             prounds.append((na, nb, recs))
-        oorder = sorted(range(len(outs)), key=lambda i: len(outs[i].t))
+        oorder = sorted((i for i in range(len(outs)) if outs[i] is not None), key=lambda i: len(outs[i].t))
         orounds = []
         for r0 in range(0, len(oorder), G):
             chunk = oorder[r0:r0 + G]
@@ -581,7 +642,7 @@ def main():
              "namespace sched {"]
     for k, (name, prods, outs, ro) in enumerate(progs):
         lines.append("constexpr int %s = %d;  // %d products, %d outputs%s" %
-                     (name, k, len(prods), len(outs), ", outputs read other inputs" if ro else ""))
+                     (name, k, len(prods), sum(o is not None for o in outs), ", outputs read other inputs" if ro else ""))
     lines.append("constexpr int NUM_OPS = %d;" % len(progs))
     lines.append("constexpr int NUM_CONSTS = %d;" % len(CONSTS.vals))
     lines.append("constexpr int PROD_Q = %d;  // 16-byte units per product record" % PROD_Q)
@@ -608,7 +669,8 @@ def main():
     with open(os.path.join(HERE, "schedules_gen.hpp"), "w") as f:
         f.write("\n".join(lines) + "\n")
     for name, prods, outs, ro in progs:
-        print("%-12s products %3d outputs %3d%s" % (name, len(prods), len(outs), "  (reads other inputs)" if ro else ""))
+        print("%-12s products %3d outputs %3d%s" % (name, len(prods), sum(o is not None for o in outs),
+                                                   "  (reads other inputs)" if ro else ""))
     for G, (nb, nr) in sizes.items():
         print("G=%2d: %4d rounds, %6d bytes of records" % (G, nr, nb))
     print("constants", len(CONSTS.vals))

@@ -200,8 +200,10 @@ __device__ __forceinline__ void load_consts(int K) {
 
 // ---------------- Miller loop ----------------
 // slots per item: F 0..11 | L1 12..17 | L2 18..23 | P 24..27 (P1.x P1.y P2.x P2.y) |
-// S1 28..33 | S2 34..39 | T 40..78; the constants follow the last item.
-constexpr int ML_SLOTS = 79;
+// S1 28..33 | S2 34..39 | T 40..98; the constants follow the last item.  The merged line
+// ops address L1 .. S2 as one region (B 0..27) and write S1 | S2 at f + 28 .. f + 39.
+constexpr int ML_SLOTS = 99;
+static_assert(ML_SLOTS - 40 >= 59, "product slots of OP_FL_SCALE");
 
 template <int G>
 constexpr size_t miller_lds_bytes() {
@@ -259,9 +261,9 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
   const i32* pk = key_idx ? keys + (size_t)kidx * KEY_TABLE_WORDS : nullptr;
   __syncthreads();
 
-  auto step = [&](int ci) {
-    // lines of step ci: generator (same for every item) and this item's key; only
-    // items that use them read them (a skipped item may have no valid key table)
+  // lines of step ci into L1 | L2: generator (same for every item) and this item's key;
+  // only items that use them read them (a skipped item may have no valid key table)
+  auto load_lines = [&](int ci) {
     if (act) {
       for (int w = sub; w < 2 * 84; w += G) {
         const int which = w / 84, ww = w % 84, e = ww / NL, l = ww % NL;
@@ -271,18 +273,43 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
         g_lds[(which == 0 ? L1 : L2) + e * SLOT + l] = v;
       }
     }
-    __syncthreads();
-    run_op<G>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);  // S1 | S2 (both pairs)
-    run_op<G>(sched::OP_ELL, sub, act1, F, S1, F, T, K);
-    run_op<G>(sched::OP_ELL, sub, act, F, S2, F, T, K);
   };
   int ci = 0;
-  for (int b = 61; b >= 0; b--) {
-    step(ci++);
-    if (miller_bit(b)) step(ci++);
-    run_op<G>(sched::OP_SQR, sub, act, F, 0, F, T, K);
+  if (__ballot(act && !act1) == 0) {
+    // Every item of the tile pairs both pairs: each step multiplies f by the product of
+    // its two lines (OP_LL, then OP_FL_SCALE), and the scaling of the next step's lines
+    // rides along with f * LL.  Region B of both ops is the line block L1 | L2 | P | S.
+    load_lines(0);
+    __syncthreads();
+    run_op<G>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);
+    auto mstep = [&](bool last) {
+      if (!last) load_lines(ci + 1);  // L1 | L2 are free once S holds their scaled form
+      run_op<G>(sched::OP_LL, sub, act, 0, L1, S1, T, K);
+      run_op<G>(last ? sched::OP_FL : sched::OP_FL_SCALE, sub, act, F, L1, F, T, K);
+      ci++;
+    };
+    for (int b = 61; b >= 0; b--) {
+      mstep(false);
+      if (miller_bit(b)) mstep(false);
+      run_op<G>(sched::OP_SQR, sub, act, F, 0, F, T, K);
+    }
+    mstep(true);
+  } else {
+    // a tile with an identity signature: its pair-1 lines are skipped item by item
+    auto step = [&]() {
+      load_lines(ci++);
+      __syncthreads();
+      run_op<G>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);  // S1 | S2 (both pairs)
+      run_op<G>(sched::OP_ELL, sub, act1, F, S1, F, T, K);
+      run_op<G>(sched::OP_ELL, sub, act, F, S2, F, T, K);
+    };
+    for (int b = 61; b >= 0; b--) {
+      step();
+      if (miller_bit(b)) step();
+      run_op<G>(sched::OP_SQR, sub, act, F, 0, F, T, K);
+    }
+    step();
   }
-  step(ci);
   run_op<G>(sched::OP_CONJ, sub, act, F, 0, F, T, K);
   if (valid && act) {
     for (int w = sub; w < 12 * NL; w += G) {

@@ -57,7 +57,7 @@ def run_op(rounds, a, b):
             assert all(decode(t)[2] == 0 for t in rec[1 + n:])
             assert rec[0] not in out
             out[rec[0]] = sum(val(t) for t in rec[1:1 + n]) % P
-    return [out[i] for i in range(len(out))]
+    return [out.get(i) for i in range(max(out) + 1)]
 
 
 def symbolic(prods, outs, a, b):
@@ -70,7 +70,7 @@ def symbolic(prods, outs, a, b):
         return s % P
 
     tv = [ev(x, None) * ev(y, None) % P for x, y in prods]
-    return [ev(o, tv) for o in outs]
+    return [None if o is None else ev(o, tv) for o in outs]
 
 
 @pytest.fixture(scope="module")
@@ -90,10 +90,10 @@ def test_round_tables_execute_every_op(progs, G):
             busy = [r[0] != GSCH.IDLE for r in recs]
             assert busy == sorted(busy, reverse=True)
         assert sum(1 for _, _, recs in prounds for r in recs if r[0] != GSCH.IDLE) == len(prods)
-        assert sum(1 for _, _, recs in orounds for r in recs if r[0] != GSCH.IDLE) == len(outs)
+        assert sum(1 for _, _, recs in orounds for r in recs if r[0] != GSCH.IDLE) == sum(o is not None for o in outs)
         for _ in range(3):
             a = [rnd.randrange(P) for _ in range(12)]
-            b = [rnd.randrange(P) for _ in range(12)]
+            b = [rnd.randrange(P) for _ in range(28)]
             assert run_op((prounds, orounds), a, b) == symbolic(prods, outs, a, b), name
 
 
@@ -118,6 +118,39 @@ def test_fp12_mul_and_square_against_schoolbook(progs, G):
         assert run_op(mul, a, b) == flat(want)
         want2 = PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(a)), PF.f12_from_tower(tower(a))))
         assert run_op(sqr, a, [0] * 12) == flat(want2)
+
+
+@pytest.mark.parametrize("G", [12, 16, 20])
+def test_merged_line_step_against_schoolbook(progs, G):
+    """LL = line1 * line2 and f * LL (with the next step's scaled lines) equal the full
+    Fp12 products of the sparse lines x0 + x1 v + y1 v w"""
+    rnd = random.Random(200 + G)
+    ops = {name: (p, o) for name, p, o in GSCH.build_rounds(G, progs)}
+    zero = (0, 0)
+    for _ in range(3):
+        b = [rnd.randrange(P) for _ in range(28)]
+        s1 = [(b[16 + 2 * k], b[17 + 2 * k]) for k in range(3)]
+        s2 = [(b[22 + 2 * k], b[23 + 2 * k]) for k in range(3)]
+        l1 = [s1[0], s1[1], zero, zero, s1[2], zero]  # tower order c0.c0 c0.c1 c0.c2 c1.c0 c1.c1 c1.c2
+        l2 = [s2[0], s2[1], zero, zero, s2[2], zero]
+        want = PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(l1), PF.f12_from_tower(l2)))
+        got = run_op(ops["OP_LL"], [0] * 12, b)
+        assert want[3] == zero
+        assert got == flat([want[0], want[1], want[2], want[4], want[5]])
+        # f * LL, LL written over the scaled lines (B 16..25)
+        b2 = list(b)
+        b2[16:26] = got
+        f = [rnd.randrange(P) for _ in range(12)]
+        prod = flat(PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(f)), PF.f12_from_tower(want))))
+        out = run_op(ops["OP_FL_SCALE"], f, b2)
+        assert out[:12] == prod and out[12:28] == [None] * 16
+        # the scaled lines of the next step: (c2, c1 P.x, c0 P.y) per pair
+        for pair in range(2):
+            c = [(b2[6 * pair + 2 * k], b2[6 * pair + 2 * k + 1]) for k in range(3)]
+            x, y = b2[12 + 2 * pair], b2[13 + 2 * pair]
+            sc = [c[2], (c[1][0] * x % P, c[1][1] * x % P), (c[0][0] * y % P, c[0][1] * y % P)]
+            assert out[28 + 6 * pair:34 + 6 * pair] == flat(sc)
+        assert run_op(ops["OP_FL"], f, b2) == prod
 
 
 def test_generated_header_is_current(progs):
