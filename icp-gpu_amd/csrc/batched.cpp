@@ -352,7 +352,7 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
     *ms = best;
     return 0;
   };
-  const int gml[] = {8, 12, 13, 16, 20, 32}, gfe[] = {9, 12, 16, 18, 27, 32};
+  const int gml[] = {8, 12, 13, 16, 20, 24, 32}, gfe[] = {9, 12, 16, 18, 24, 27, 32};
   // 1. the final exponentiation alone, on the Miller output of the current configuration
   CHK(bls::engine_miller(a, g.s));
   double best_ml = 1e30, best_fe = 1e30;

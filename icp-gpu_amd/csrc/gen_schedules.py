@@ -517,7 +517,7 @@ OPS = [
 # padded to 24 halfwords; output rounds [dst, terms] padded to a multiple of 8.
 # dst = 0xffff marks an idle lane.  Lane s of the group reads the record at
 # desc + s * q (in 16-byte units).
-GS = [8, 9, 12, 13, 16, 18, 20, 27, 32]
+GS = [8, 9, 12, 13, 16, 18, 20, 24, 27, 32]
 IDLE = 0xFFFF
 PROD_Q = 3        # 24 halfwords per product record
 B_AT = 9          # first B term of a product record

@@ -23,9 +23,10 @@ int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
                      int block, hipStream_t s);
 
-// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13, 16,
-// 20, 32; final exponentiation: 9, 12, 16, 18, 27, 32).  Return -1 for an unsupported G or a launch error.
-// queue == nullptr: one workgroup per tile of 64/G items.  Otherwise a persistent grid
+// lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13, 16, 20,
+// 24, 32; final exponentiation: 9, 12, 16, 18, 24, 27, 32; 24 and 32 run two wavefronts per
+// workgroup).  Return -1 for an unsupported G or a launch error.
+// queue == nullptr: one workgroup per tile of its items.  Otherwise a persistent grid
 // (at most wg_per_cu workgroups per CU, 0 = the occupancy limit) pops tiles from the
 // device counter *queue, which must be zero at launch.
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,

@@ -1,13 +1,16 @@
 // Lanes-per-pairing kernels: a group of G lanes of one wavefront verifies one item.
 //
-// Each item owns a block of Fp "slots" in LDS (SLOT = 16 words; 14 limbs used).  Every
+// Each item owns a block of Fp "slots" in LDS (SLOT = 14 words, one per limb).  Every
 // extension-field operation is a bilinear schedule (schedules_gen.hpp): the group's
 // lanes split its independent Fp products, then its output combinations.  The only
 // multiplier body in the kernel is the one inside run_op(), so the code stays small
 // enough for the instruction cache and needs no calls or scratch.
 //
-// Workgroups are one wavefront (64 lanes) holding 64 / G items; lanes past the last
-// full group idle but take part in the barriers.  The Miller value passes between the
+// A workgroup is W wavefronts holding 64 / L items, L = G / W lanes of each wavefront
+// per item (Cfg below): one wavefront for G <= 27; two for G = 24 and 32, so that two
+// wavefronts share each SIMD when the items exactly fill the GPU at one workgroup per
+// tile (a lone wavefront issues at most one instruction per ~6 cycles, two together
+// one per ~4.9).  Lanes past the last full group idle but take part in the barriers.  The Miller value passes between the
 // two kernels item-major (168 consecutive words per item), so the G lanes of an item
 // read and write whole cache lines whichever XCD runs the workgroup.
 #include <hip/hip_runtime.h>
@@ -18,29 +21,43 @@
 
 namespace bls {
 
-constexpr int SLOT = 16;
+constexpr int SLOT = 14;
+
+// Layout of a lanes-per-item configuration G: W wavefronts per workgroup, L = G / W lanes
+// of each wavefront per item, IPW = 64 / L items per workgroup.  Lane s (0 <= s < G) of
+// an item sits in wavefront s / L at position item * L + s % L.
+template <int G>
+struct Cfg {
+  static constexpr int W = (G == 24 || G == 32) ? 2 : 1;
+  static constexpr int L = G / W;
+  static constexpr int IPW = 64 / L;
+  static constexpr int THREADS = 64 * W;
+  static_assert(L * W == G, "lanes per item");
+};
 
 // All LDS of these kernels is one dynamic array; device functions address it by word
 // offset so that they stay in the LDS address space across calls.
 extern __shared__ __align__(16) i32 g_lds[];
 
+// Slots are 56 bytes apart, so the slots that the lanes of one LDS access read start at
+// 8-byte offsets spread over the banks instead of all at 64-byte boundaries (with
+// 64-byte slots, lanes reading different slots collided on the same banks); a slot moves
+// as three ds_read2_b64 / ds_write2_b64 and one 8-byte access.
 __device__ __forceinline__ Fp lds_ld(int off) {
-  const int4* q = reinterpret_cast<const int4*>(g_lds + off);
-  int4 a = q[0], b = q[1], c = q[2];
-  int2 d = reinterpret_cast<const int2*>(g_lds + off)[6];
+  const int2* q = reinterpret_cast<const int2*>(g_lds + off);
   Fp r;
-  r.l[0] = a.x; r.l[1] = a.y; r.l[2] = a.z; r.l[3] = a.w;
-  r.l[4] = b.x; r.l[5] = b.y; r.l[6] = b.z; r.l[7] = b.w;
-  r.l[8] = c.x; r.l[9] = c.y; r.l[10] = c.z; r.l[11] = c.w;
-  r.l[12] = d.x; r.l[13] = d.y;
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    const int2 v = q[j];
+    r.l[2 * j] = v.x;
+    r.l[2 * j + 1] = v.y;
+  }
   return r;
 }
 __device__ __forceinline__ void lds_st(int off, const Fp& v) {
-  int4* q = reinterpret_cast<int4*>(g_lds + off);
-  q[0] = make_int4(v.l[0], v.l[1], v.l[2], v.l[3]);
-  q[1] = make_int4(v.l[4], v.l[5], v.l[6], v.l[7]);
-  q[2] = make_int4(v.l[8], v.l[9], v.l[10], v.l[11]);
-  reinterpret_cast<int2*>(g_lds + off)[6] = make_int2(v.l[12], v.l[13]);
+  int2* q = reinterpret_cast<int2*>(g_lds + off);
+#pragma unroll
+  for (int j = 0; j < 7; j++) q[j] = make_int2(v.l[2 * j], v.l[2 * j + 1]);
 }
 
 __device__ __forceinline__ void st_ref_words(u64* p, const Fp& a) {
@@ -143,6 +160,9 @@ __device__ __forceinline__ P* uniform_ptr(P* p) {
 // lane of the wavefront runs the same instruction stream whatever its record holds.
 // D may alias A (every op used in place reads no other input component in its
 // outputs).  Not inlined: one copy of the multiplier serves every call site.
+// One instantiation per wavefronts-per-workgroup count, so that the two-wavefront kernels'
+// register budget (two waves per SIMD) applies to their copy of the interpreter.
+template <int WAVES>
 __device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* rounds, const uint4* __restrict__ desc,
                                          int op, int sub, bool active, int A, int B, int D, int T, int K) {
   // the table pointers, the op and the round shapes are the same on every lane: make
@@ -161,20 +181,26 @@ __device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* r
     const int na = rounds[first + r].na, nb = rounds[first + r].nb;
     if (r + 1 < nr) load_record(nxt, desc, rounds[first + r + 1].desc, rounds[first + r + 1].q, sub);
     const u32 dst = rec_hw(cur, 0);
+    // a wavefront none of whose lanes has a product or output in this round skips it
+    // (the records of a round list the busy lanes first, so with two wavefronts per
+    // item the second one is often idle)
+    const bool work = __ballot(dst != sched::IDLE) != 0;
     i64 acc[NL];
 #pragma unroll
     for (int i = 0; i < NL; i++) acc[i] = 0;
     if (r < np) {
-      accumulate<1, 8>(acc, cur, na, R);
-      const Fp a = carry64(acc);
+      if (work) {
+        accumulate<1, 8>(acc, cur, na, R);
+        const Fp a = carry64(acc);
 #pragma unroll
-      for (int i = 0; i < NL; i++) acc[i] = 0;
-      accumulate<sched::B_AT, 8>(acc, cur, nb, R);
-      const Fp b = carry64(acc);
-      const Fp p = mul(a, b);
-      if (active && dst != sched::IDLE) lds_st(T + (int)dst * SLOT, p);
+        for (int i = 0; i < NL; i++) acc[i] = 0;
+        accumulate<sched::B_AT, 8>(acc, cur, nb, R);
+        const Fp b = carry64(acc);
+        const Fp p = mul(a, b);
+        if (active && dst != sched::IDLE) lds_st(T + (int)dst * SLOT, p);
+      }
       if (r + 1 == np) __syncthreads();
-    } else {
+    } else if (work) {
       accumulate<1, 2 * REC_WORDS - 1>(acc, cur, na, R);
       const Fp o = fold_top(carry64(acc));
       if (active && dst != sched::IDLE) lds_st(D + (int)dst * SLOT, o);
@@ -188,7 +214,8 @@ __device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* r
 }
 template <int G>
 __device__ __forceinline__ void run_op(int op, int sub, bool active, int A, int B, int D, int T, int K) {
-  run_op_impl(sched::Tab<G>::ops(), sched::Tab<G>::rounds(), sched::Tab<G>::desc(), op, sub, active, A, B, D, T, K);
+  run_op_impl<Cfg<G>::W>(sched::Tab<G>::ops(), sched::Tab<G>::rounds(), sched::Tab<G>::desc(), op, sub, active, A, B, D,
+                         T, K);
 }
 
 __device__ __forceinline__ void load_consts(int K) {
@@ -207,7 +234,7 @@ static_assert(ML_SLOTS - 40 >= 59, "product slots of OP_FL_SCALE");
 
 template <int G>
 constexpr size_t miller_lds_bytes() {
-  return (size_t)((64 / G) * ML_SLOTS + sched::NUM_CONSTS) * SLOT * 4;
+  return (size_t)(Cfg<G>::IPW * ML_SLOTS + sched::NUM_CONSTS) * SLOT * 4;
 }
 
 template <int G>
@@ -216,8 +243,8 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
                                             const int32_t* __restrict__ key_idx, int nkeys,
                                             const i32* __restrict__ pk_soa, i32* __restrict__ status,
                                             i32* __restrict__ fout, int M) {
-  constexpr int IPW = 64 / G;
-  const int lid = threadIdx.x / G, sub = threadIdx.x % G;
+  constexpr int IPW = Cfg<G>::IPW, L = Cfg<G>::L;
+  const int ln = threadIdx.x & 63, lid = ln / L, sub = (int)(threadIdx.x >> 6) * L + ln % L;
   const bool in_wg = lid < IPW;
   const int item = tile * IPW + lid;
   const bool valid = in_wg && item < M;
@@ -275,7 +302,7 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
     }
   };
   int ci = 0;
-  if (__ballot(act && !act1) == 0) {
+  if (!__syncthreads_or(act && !act1)) {
     // Every item of the tile pairs both pairs: each step multiplies f by the product of
     // its two lines (OP_LL, then OP_FL_SCALE), and the scaling of the next step's lines
     // rides along with f * LL.  Region B of both ops is the line block L1 | L2 | P | S.
@@ -334,12 +361,12 @@ __device__ __forceinline__ int next_tile(int* queue, int round) {
 }
 
 template <int G>
-__global__ __launch_bounds__(64) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
+__global__ __launch_bounds__(Cfg<G>::THREADS, Cfg<G>::W) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                                    const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                                    const int32_t* __restrict__ key_idx, int nkeys,
                                                    const i32* __restrict__ pk_soa, i32* __restrict__ status,
                                                    i32* __restrict__ fout, int M, int* queue) {
-  constexpr int IPW = 64 / G;
+  constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * ML_SLOTS * SLOT;
   // raised wave priority: a concurrent subgroup-test wave on the same SIMD issues in the
@@ -360,7 +387,7 @@ constexpr int FE_SLOTS = 126;
 
 template <int G>
 constexpr size_t final_exp_lds_bytes() {
-  return ((size_t)((64 / G) * FE_SLOTS + sched::NUM_CONSTS) * SLOT + (64 / G) * 12) * 4;
+  return ((size_t)(Cfg<G>::IPW * FE_SLOTS + sched::NUM_CONSTS) * SLOT + Cfg<G>::IPW * 12) * 4;
 }
 
 template <int G>
@@ -377,8 +404,8 @@ template <int G>
 __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__ fin, const i32* __restrict__ status,
                                                const i32* __restrict__ sub_status, int32_t* __restrict__ result,
                                                u64* __restrict__ gt_out, int M) {
-  constexpr int IPW = 64 / G;
-  const int lid = threadIdx.x / G, sub = threadIdx.x % G;
+  constexpr int IPW = Cfg<G>::IPW, L = Cfg<G>::L;
+  const int ln = threadIdx.x & 63, lid = ln / L, sub = (int)(threadIdx.x >> 6) * L + ln % L;
   const bool in_wg = lid < IPW;
   const int item = tile * IPW + lid;
   const bool valid = in_wg && item < M;
@@ -488,10 +515,10 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
 }
 
 template <int G>
-__global__ __launch_bounds__(64) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
+__global__ __launch_bounds__(Cfg<G>::THREADS, Cfg<G>::W) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
                                                       const i32* __restrict__ sub_status, int32_t* __restrict__ result,
                                                       u64* __restrict__ gt_out, int M, int* queue) {
-  constexpr int IPW = 64 / G;
+  constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * FE_SLOTS * SLOT;
   __builtin_amdgcn_s_setprio(2);
@@ -517,13 +544,13 @@ static int num_cus() {
 template <int G>
 static int ml_occ() {
   static int occ = 0;
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G>, 64, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G>, Cfg<G>::THREADS, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
 }
 template <int G>
 static int fe_occ() {
   static int occ = 0;
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G>, 64, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G>, Cfg<G>::THREADS, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
 }
 // Grid of a queue-fed launch: at most wg_per_cu workgroups per CU (0 = as many as
@@ -538,20 +565,20 @@ template <int G>
 static int launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
                      int* queue, int wg_per_cu, hipStream_t s) {
-  constexpr int IPW = 64 / G;
+  constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   const int grid = queue ? queue_grid(ml_occ<G>(), ntiles, wg_per_cu) : ntiles;
-  hipLaunchKernelGGL(k_miller_grp<G>, dim3(grid), dim3(64), miller_lds_bytes<G>(), s, sig, nhm, gen, keys, key_idx,
+  hipLaunchKernelGGL(k_miller_grp<G>, dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen, keys, key_idx,
                      nkeys, pk_soa, status, fout, M, queue);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int G>
 static int launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
                      uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
-  constexpr int IPW = 64 / G;
+  constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   const int grid = queue ? queue_grid(fe_occ<G>(), ntiles, wg_per_cu) : ntiles;
-  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3(grid), dim3(64), final_exp_lds_bytes<G>(), s, fin, status, sub_status,
+  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3(grid), dim3(Cfg<G>::THREADS), final_exp_lds_bytes<G>(), s, fin, status, sub_status,
                      result, gt, M, queue);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -562,7 +589,7 @@ int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32
   switch (G) {
 #define ML_CASE(g) \
   case g: return launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, s);
-    ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(20) ML_CASE(32)
+    ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(20) ML_CASE(24) ML_CASE(32)
 #undef ML_CASE
     default: return -1;
   }
@@ -571,15 +598,15 @@ int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const
                          uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
   switch (G) {
 #define FE_CASE(g) case g: return launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, s);
-    FE_CASE(9) FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(27) FE_CASE(32)
+    FE_CASE(9) FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(24) FE_CASE(27) FE_CASE(32)
 #undef FE_CASE
     default: return -1;
   }
 }
 
 bool grp_lanes_supported(int which, int G) {
-  if (which == 0) return G == 8 || G == 12 || G == 13 || G == 16 || G == 20 || G == 32;
-  return G == 9 || G == 12 || G == 16 || G == 18 || G == 27 || G == 32;
+  if (which == 0) return G == 8 || G == 12 || G == 13 || G == 16 || G == 20 || G == 24 || G == 32;
+  return G == 9 || G == 12 || G == 16 || G == 18 || G == 24 || G == 27 || G == 32;
 }
 
 int occupancy_grp(int which, int G) {
@@ -590,6 +617,7 @@ int occupancy_grp(int which, int G) {
       case 13: return ml_occ<13>();
       case 16: return ml_occ<16>();
       case 20: return ml_occ<20>();
+      case 24: return ml_occ<24>();
       case 32: return ml_occ<32>();
       default: return -1;
     }
@@ -599,6 +627,7 @@ int occupancy_grp(int which, int G) {
     case 12: return fe_occ<12>();
     case 16: return fe_occ<16>();
     case 18: return fe_occ<18>();
+    case 24: return fe_occ<24>();
     case 27: return fe_occ<27>();
     case 32: return fe_occ<32>();
     default: return -1;

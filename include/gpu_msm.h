@@ -127,8 +127,8 @@ int blsgpu_verify_batches_device(const uint8_t* d_sigs48, const uint64_t* d_neg_
 int blsgpu_verify_batches(const uint8_t* const* sigs48, const uint64_t* const* neg_hms,
                           const int32_t* const* key_idx, const int32_t* sizes, int nbatches,
                           int32_t* const* results, uint64_t* const* gt_out);
-/* Batched-form configuration: lanes per item of the Miller loop (8, 12, 13, 16, 20, 32) and of
- * the final exponentiation (9, 12, 16, 18, 27, 32), workgroups per CU of each (0 = occupancy
+/* Batched-form configuration: lanes per item of the Miller loop (8, 12, 13, 16, 20, 24, 32) and of
+ * the final exponentiation (9, 12, 16, 18, 24, 27, 32), workgroups per CU of each (0 = occupancy
  * limit, -1 = one workgroup per tile, no queue), and graph capture on/off.  Clears the
  * autotuned table. */
 int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int wpc_final_exp, int use_graph);
@@ -154,7 +154,7 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
 
 /* Engine selection: mode 0 = one thread per verification (thread-per-item kernels),
  * mode 1 = a group of G lanes of one wavefront per verification (g_miller in
- * {8, 12, 13, 16, 20, 32} for the Miller loop, g_final_exp in {9, 12, 16, 18, 27, 32} for the final
+ * {8, 12, 13, 16, 20, 24, 32} for the Miller loop, g_final_exp in {9, 12, 16, 18, 24, 27, 32} for the final
  * exponentiation).  The
  * default is mode 1 with 12/16. */
 int blsgpu_set_engine(int mode, int g_miller, int g_final_exp);

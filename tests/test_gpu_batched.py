@@ -137,7 +137,7 @@ def test_autotuner_picks_a_valid_config_and_stays_exact(small):
     rc = blsgpu.load().blsgpu_autotune_device(sig.data_ptr(), nhm.data_ptr(), ki.data_ptr(), 97, 0,
                                               torch.cuda.current_stream().cuda_stream, O.ptr(cfg), O.ptr(ms))
     assert rc == 0, blsgpu.last_error()
-    assert cfg[0] in (8, 12, 13, 16, 20, 32) and cfg[1] in (9, 12, 16, 18, 27, 32)
+    assert cfg[0] in (8, 12, 13, 16, 20, 24, 32) and cfg[1] in (9, 12, 16, 18, 24, 27, 32)
     assert (ms > 0).all()
     assert blsgpu.get_batched_engine(97)[:4] == tuple(int(c) for c in cfg)
     res2, gt2 = _call_device((sig, nhm, ki, res, gt), _split(97, [97]))

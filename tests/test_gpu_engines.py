@@ -9,7 +9,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = [(1, 16, 16), (1, 8, 16), (1, 32, 32), (1, 12, 12), (1, 13, 18), (1, 20, 27), (1, 16, 9), (0, 16, 16)]
+CONFIGS = [(1, 16, 16), (1, 8, 16), (1, 32, 32), (1, 12, 12), (1, 13, 18), (1, 20, 27), (1, 16, 9), (1, 24, 24), (0, 16, 16)]
 
 
 @pytest.fixture(scope="module")
