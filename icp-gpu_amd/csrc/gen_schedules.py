@@ -210,8 +210,12 @@ def f12mul_by_014(R, f, c0, c1, c4):
 
 
 def fp4sqr(R, a, b):
+    """(a + b s)^2 with s^2 = 1 + u: (a^2 + (1+u) b^2, 2ab).  2ab is taken as a product
+    (three Fp products whose operands have at most two terms) rather than as
+    (a + b)^2 - a^2 - b^2 (two products of four-term operands): one more product, but
+    half the operand gathering in the rounds of the lanes-per-item kernels"""
     t0, t1 = f2sqr(R, a), f2sqr(R, b)
-    return t1.nr() + t0, f2sqr(R, a + b) - t0 - t1
+    return t1.nr() + t0, f2mul(R, a, b).scale(2)
 
 
 def cyc_sqr(R, f):

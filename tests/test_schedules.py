@@ -153,6 +153,25 @@ def test_merged_line_step_against_schoolbook(progs, G):
         assert run_op(ops["OP_FL"], f, b2) == prod
 
 
+def test_cyclotomic_square_on_a_cyclotomic_element(progs):
+    """the cyclotomic-square schedule equals the full square on an element of the
+    cyclotomic subgroup, f^((p^6 - 1)(p^2 + 1)) for a random f (and differs off it)"""
+    rnd = random.Random(7)
+    f = [(rnd.randrange(P), rnd.randrange(P)) for _ in range(6)]
+    fw = PF.f12_from_tower(f)
+    # f^(p^6 - 1) = conj(f) / f, then raised to p^2 + 1
+    conj = [fw[i] if i % 2 == 0 else ((-fw[i][0]) % P, (-fw[i][1]) % P) for i in range(6)]
+    inv = PF.f12pow_w(fw, P ** 12 - 2)
+    g = PF.f12mul_w(conj, inv)
+    g = PF.f12mul_w(PF.f12pow_w(g, P * P), g)
+    gt = flat(PF.f12_to_tower(g))
+    want = flat(PF.f12_to_tower(PF.f12mul_w(g, g)))
+    for G in (12, 16, 32):
+        ops = {name: (p, o) for name, p, o in GSCH.build_rounds(G, progs)}
+        assert run_op(ops["OP_CYC_SQR"], gt, [0] * 28) == want
+    assert run_op(ops["OP_CYC_SQR"], flat(f), [0] * 28) != flat(PF.f12_to_tower(PF.f12mul_w(fw, fw)))
+
+
 def test_generated_header_is_current(progs):
     """schedules_gen.hpp is what the generator writes now (the build does not regenerate it)"""
     path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
