@@ -113,7 +113,16 @@ class F2:
         return F2(self.c0, -self.c1)
 
 
+# When set, an Fp2 product whose Karatsuba sum operand would have more than this many
+# terms is taken schoolbook (four products of the shorter operands).  The round tables
+# of each lane count use whichever form of an op needs fewer rounds, then fewer terms.
+CAP_TERMS = None
+
+
 def f2mul(R, a, b):
+    if CAP_TERMS is not None and (len((a.c0 + a.c1).t) > CAP_TERMS or len((b.c0 + b.c1).t) > CAP_TERMS):
+        t00, t11 = R.mul(a.c0, b.c0), R.mul(a.c1, b.c1)
+        return F2(t00 - t11, R.mul(a.c0, b.c1) + R.mul(a.c1, b.c0))
     t0 = R.mul(a.c0, b.c0)
     t1 = R.mul(a.c1, b.c1)
     s = R.mul(a.c0 + a.c1, b.c0 + b.c1)
@@ -542,12 +551,18 @@ def weight(lin):
     return sum(abs(c) for c in lin.t.values())
 
 
-def op_programs():
+def op_programs(cap=None):
     """[(name, products [(A Lin, B Lin)], outputs [Lin])], products in T order, with
-    the operand-weight bounds the kernel's arithmetic relies on checked"""
+    the operand-weight bounds the kernel's arithmetic relies on checked; cap = the
+    Karatsuba operand cap of f2mul (None: plain Karatsuba everywhere)"""
+    global CAP_TERMS
     progs = []
     for name, fn in OPS:
-        R, outs = fn()
+        CAP_TERMS = cap
+        try:
+            R, outs = fn()
+        finally:
+            CAP_TERMS = None
         for a, b in R.products:
             # operands: at most 8 terms and weight 16 (|value| < 2^386.4 keeps the
             # Montgomery product N1); outputs: weight < 2^12 (64-bit accumulation)
@@ -566,6 +581,31 @@ def op_programs():
 def build_rounds(G, progs):
     """per-op round lists for a group of G lanes:
     [(name, prod_rounds, out_rounds)], a round = (na, nb, [per-lane record halfwords])"""
+    return [(n, p, o) for n, p, o, _, _ in build_rounds_v(G, [progs])]
+
+
+def round_cost(prounds, orounds):
+    """rounds first, then operand and output terms per lane"""
+    return (len(prounds), len(orounds), sum(na + nb for na, nb, _ in prounds) + sum(n for n, _, _ in orounds))
+
+
+def build_rounds_v(G, variants):
+    """like build_rounds, choosing per op the cheapest of several programs (same op
+    list, e.g. with and without the Karatsuba cap): [(name, prod_rounds, out_rounds,
+    products, outputs)]"""
+    best = None
+    for progs in variants:
+        ops = build_rounds_one(G, progs)
+        if best is None:
+            best = [(n, p, o, prods, outs) for (n, p, o), (_, prods, outs, _) in zip(ops, progs)]
+            continue
+        for k, ((n, p, o), (_, prods, outs, _)) in enumerate(zip(ops, progs)):
+            if round_cost(p, o) < round_cost(best[k][1], best[k][2]):
+                best[k] = (n, p, o, prods, outs)
+    return best
+
+
+def build_rounds_one(G, progs):
     ops = []
     for name, prods, outs, _ in progs:
         order = sorted(range(len(prods)), key=lambda j: (len(prods[j][0].t) + len(prods[j][1].t),
@@ -640,6 +680,7 @@ def emit_tables(G, ops):
 
 def main():
     progs = op_programs()
+    variants = [progs, op_programs(cap=4)]
     lines = ["// Generated by gen_schedules.py -- do not edit.",
              "// Fp12-level operations as per-G round tables (see gen_schedules.py for the encoding).",
              "#pragma once", "#include <hip/hip_runtime.h>", "#include <stdint.h>", "namespace bls {",
@@ -648,6 +689,8 @@ def main():
         lines.append("constexpr int %s = %d;  // %d products, %d outputs%s" %
                      (name, k, len(prods), sum(o is not None for o in outs), ", outputs read other inputs" if ro else ""))
     lines.append("constexpr int NUM_OPS = %d;" % len(progs))
+    max_t = max(len(prods) for G in GS for _, _, _, prods, _ in build_rounds_v(G, variants))
+    lines.append("constexpr int MAX_PRODUCTS = %d;  // T slots an item needs (any op, any G)" % max_t)
     lines.append("constexpr int NUM_CONSTS = %d;" % len(CONSTS.vals))
     lines.append("constexpr int PROD_Q = %d;  // 16-byte units per product record" % PROD_Q)
     lines.append("constexpr int B_AT = %d;    // halfword index of the first B term" % B_AT)
@@ -665,7 +708,7 @@ def main():
     lines.append("__device__ const int32_t KCONST[%d][14] = {%s};" % (max(1, len(CONSTS.vals)), ", ".join(kvals)))
     sizes = {}
     for G in GS:
-        ops = build_rounds(G, progs)
+        ops = [(n, p, o) for n, p, o, _, _ in build_rounds_v(G, variants)]
         t, nbytes = emit_tables(G, ops)
         lines += t
         sizes[G] = (nbytes, sum(len(p) + len(o) for _, p, o in ops))

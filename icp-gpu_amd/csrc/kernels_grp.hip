@@ -227,10 +227,9 @@ __device__ __forceinline__ void load_consts(int K) {
 
 // ---------------- Miller loop ----------------
 // slots per item: F 0..11 | L1 12..17 | L2 18..23 | P 24..27 (P1.x P1.y P2.x P2.y) |
-// S1 28..33 | S2 34..39 | T 40..98; the constants follow the last item.  The merged line
+// S1 28..33 | S2 34..39 | T 40..; the constants follow the last item.  The merged line
 // ops address L1 .. S2 as one region (B 0..27) and write S1 | S2 at f + 28 .. f + 39.
-constexpr int ML_SLOTS = 99;
-static_assert(ML_SLOTS - 40 >= 59, "product slots of OP_FL_SCALE");
+constexpr int ML_SLOTS = 40 + sched::MAX_PRODUCTS;
 
 template <int G>
 constexpr size_t miller_lds_bytes() {
@@ -381,9 +380,9 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, Cfg<G>::W) void k_miller_grp(const
 }
 
 // ---------------- final exponentiation ----------------
-// slots per item: R0..R5 (12 each) 0..71 | T 72..125; then the constants, then the
-// per-item identity flags (12 words per item).
-constexpr int FE_SLOTS = 126;
+// slots per item: R0..R5 (12 each) 0..71 | T 72..; then the constants, then the per-item
+// identity flags (12 words per item).
+constexpr int FE_SLOTS = 72 + sched::MAX_PRODUCTS;
 
 template <int G>
 constexpr size_t final_exp_lds_bytes() {

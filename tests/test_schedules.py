@@ -78,11 +78,18 @@ def progs():
     return GSCH.op_programs()
 
 
+@pytest.fixture(scope="module")
+def variants(progs):
+    """the programs the generator chooses from: plain Karatsuba and capped operands"""
+    return [progs, GSCH.op_programs(cap=4)]
+
+
 @pytest.mark.parametrize("G", GSCH.GS)
-def test_round_tables_execute_every_op(progs, G):
+def test_round_tables_execute_every_op(variants, G):
+    """the tables the kernels use: per op the cheaper of the two programs"""
     rnd = random.Random(G)
-    ops = GSCH.build_rounds(G, progs)
-    for (name, prods, outs, _), (name2, prounds, orounds) in zip(progs, ops):
+    ops = GSCH.build_rounds_v(G, variants)
+    for (name, prounds, orounds, prods, outs), (name2, _, _, _) in zip(ops, variants[0]):
         assert name == name2
         # every lane record of a round has the round's shape; idle lanes only at the end
         for na, nb, recs in prounds + orounds:
@@ -105,10 +112,10 @@ def flat(t):
     return [x for pair in t for x in pair]
 
 
-@pytest.mark.parametrize("G", [13, 16, 18])
-def test_fp12_mul_and_square_against_schoolbook(progs, G):
+@pytest.mark.parametrize("G", [12, 13, 16, 18, 32])
+def test_fp12_mul_and_square_against_schoolbook(variants, G):
     rnd = random.Random(100 + G)
-    ops = {name: (p, o) for name, p, o in GSCH.build_rounds(G, progs)}
+    ops = {name: (p, o) for name, p, o, _, _ in GSCH.build_rounds_v(G, variants)}
     mul = ops["OP_MUL"]
     sqr = ops["OP_SQR"]
     for _ in range(4):
@@ -120,12 +127,12 @@ def test_fp12_mul_and_square_against_schoolbook(progs, G):
         assert run_op(sqr, a, [0] * 12) == flat(want2)
 
 
-@pytest.mark.parametrize("G", [12, 16, 20])
-def test_merged_line_step_against_schoolbook(progs, G):
+@pytest.mark.parametrize("G", [12, 16, 20, 32])
+def test_merged_line_step_against_schoolbook(variants, G):
     """LL = line1 * line2 and f * LL (with the next step's scaled lines) equal the full
     Fp12 products of the sparse lines x0 + x1 v + y1 v w"""
     rnd = random.Random(200 + G)
-    ops = {name: (p, o) for name, p, o in GSCH.build_rounds(G, progs)}
+    ops = {name: (p, o) for name, p, o, _, _ in GSCH.build_rounds_v(G, variants)}
     zero = (0, 0)
     for _ in range(3):
         b = [rnd.randrange(P) for _ in range(28)]
@@ -153,7 +160,7 @@ def test_merged_line_step_against_schoolbook(progs, G):
         assert run_op(ops["OP_FL"], f, b2) == prod
 
 
-def test_cyclotomic_square_on_a_cyclotomic_element(progs):
+def test_cyclotomic_square_on_a_cyclotomic_element(variants):
     """the cyclotomic-square schedule equals the full square on an element of the
     cyclotomic subgroup, f^((p^6 - 1)(p^2 + 1)) for a random f (and differs off it)"""
     rnd = random.Random(7)
@@ -167,16 +174,16 @@ def test_cyclotomic_square_on_a_cyclotomic_element(progs):
     gt = flat(PF.f12_to_tower(g))
     want = flat(PF.f12_to_tower(PF.f12mul_w(g, g)))
     for G in (12, 16, 32):
-        ops = {name: (p, o) for name, p, o in GSCH.build_rounds(G, progs)}
+        ops = {name: (p, o) for name, p, o, _, _ in GSCH.build_rounds_v(G, variants)}
         assert run_op(ops["OP_CYC_SQR"], gt, [0] * 28) == want
     assert run_op(ops["OP_CYC_SQR"], flat(f), [0] * 28) != flat(PF.f12_to_tower(PF.f12mul_w(fw, fw)))
 
 
-def test_generated_header_is_current(progs):
+def test_generated_header_is_current(variants):
     """schedules_gen.hpp is what the generator writes now (the build does not regenerate it)"""
     path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
     text = open(path).read()
     for G in GSCH.GS:
-        tables, nbytes = GSCH.emit_tables(G, GSCH.build_rounds(G, progs))
+        tables, nbytes = GSCH.emit_tables(G, [(n, p, o) for n, p, o, _, _ in GSCH.build_rounds_v(G, variants)])
         for line in tables:
             assert line in text, "schedules_gen.hpp is stale: run gen_schedules.py (G=%d)" % G
