@@ -15,6 +15,7 @@
 // (gpu-crypto/cuda/api.cu:128-160), which the plain form here keeps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -337,25 +338,33 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
   bls::EngineArgs a =
       batched_args(d_sigs48, d_neg_hms, d_key_idx, g.b_res.as<int32_t>(), nullptr, M, c);
   CHK(bls::engine_prepare(a, g.s));
-  // times one phase (1 final exponentiation, 2 the whole sequence) of configuration a; best of 2
+  // times one phase (1 final exponentiation, 2 the whole sequence) of configuration a: the
+  // median of three runs after one untimed run (the whole sequence overlaps two streams,
+  // and the best of a few runs can be a run where the concurrent kernel happened to
+  // interfere less than it does in steady state)
   auto time_phase = [&](int which, double* ms) -> int {
-    float best = 1e30f;
-    for (int r = 0; r < 2; r++) {
+    float t[3];
+    for (int r = 0; r < 4; r++) {
       HIPCHK(hipEventRecord(e0, g.s));
       CHK(which == 1 ? bls::engine_final_exp(a, g.s) : bls::engine_compressed(a, g.s));
       HIPCHK(hipEventRecord(e1, g.s));
       HIPCHK(hipEventSynchronize(e1));
-      float t = 0;
-      HIPCHK(hipEventElapsedTime(&t, e0, e1));
-      if (t < best) best = t;
+      if (r > 0) HIPCHK(hipEventElapsedTime(&t[r - 1], e0, e1));
     }
-    *ms = best;
+    std::sort(t, t + 3);
+    *ms = t[1];
     return 0;
   };
   const int gml[] = {8, 12, 13, 16, 20, 24, 32}, gfe[] = {9, 12, 16, 18, 24, 27, 32};
-  // 1. the final exponentiation alone, on the Miller output of the current configuration
+  // 1. the final exponentiation alone, on the Miller output of the current configuration;
+  // the fastest candidates (three for the plain form, two for the batched form, which
+  // has more workgroups-per-CU variants) go on to step 2
   CHK(bls::engine_miller(a, g.s));
-  double best_ml = 1e30, best_fe = 1e30;
+  struct Cand {
+    double ms;
+    int G, w;
+  };
+  std::vector<Cand> fe;
   for (int G : gfe) {
     const int occ = form == 0 ? bls::occupancy_grp(1, G) : -1;
     for (int w = -1; w <= occ; w++) {
@@ -364,31 +373,36 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
       a.wpc_fe = w;
       double t = 0;
       CHK(time_phase(1, &t));
-      if (t < best_fe) {
-        best_fe = t;
-        c.g_fe = G;
-        c.wpc_fe = w;
-      }
+      fe.push_back({t, G, w});
     }
   }
-  a.g_fe = c.g_fe;
-  a.wpc_fe = c.wpc_fe;
-  // 2. the Miller candidates inside the whole sequence (decode, then the Miller loop with
-  // the subgroup test running concurrently, then the final exponentiation): the
-  // concurrent test competes for the SIMDs, which favours configurations that leave
-  // some free, so the Miller loop is not timed alone
-  for (int G : gml) {
-    const int occ = form == 0 ? bls::occupancy_grp(0, G) : -1;
-    for (int w = -1; w <= occ; w++) {
-      if (w == 0) continue;
-      a.g_ml = G;
-      a.wpc_ml = w;
-      double t = 0;
-      CHK(time_phase(2, &t));
-      if (t < best_ml) {
-        best_ml = t;
-        c.g_ml = G;
-        c.wpc_ml = w;
+  std::sort(fe.begin(), fe.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+  const size_t keep = form == 0 ? 2 : 3;
+  if (fe.size() > keep) fe.resize(keep);
+  const double best_fe = fe[0].ms;
+  // 2. every Miller candidate with each of those, inside the whole sequence (decode, the
+  // Miller loop and the final exponentiation, with the subgroup test running alongside
+  // one of them): the concurrent test competes for the SIMDs, which favours pairs that
+  // leave some free, so the two kernels are not timed alone
+  double best_ml = 1e30;
+  for (const Cand& f : fe) {
+    a.g_fe = f.G;
+    a.wpc_fe = f.w;
+    for (int G : gml) {
+      const int occ = form == 0 ? bls::occupancy_grp(0, G) : -1;
+      for (int w = -1; w <= occ; w++) {
+        if (w == 0) continue;
+        a.g_ml = G;
+        a.wpc_ml = w;
+        double t = 0;
+        CHK(time_phase(2, &t));
+        if (t < best_ml) {
+          best_ml = t;
+          c.g_ml = G;
+          c.wpc_ml = w;
+          c.g_fe = f.G;
+          c.wpc_fe = f.w;
+        }
       }
     }
   }

@@ -36,6 +36,8 @@ int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const
                          uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s);
 // whether a lanes-per-item kernel (which: 0 Miller, 1 final exp) exists for G lanes
 bool grp_lanes_supported(int which, int G);
+// whether one workgroup per tile of M items at G lanes per item gives every SIMD a wavefront
+bool grp_fills_simds(int G, int M);
 // resident workgroups per CU of a grouped kernel (which: 0 Miller, 1 final exp), -1 if G is unsupported
 int occupancy_grp(int which, int G);
 // p[0..n) = 0 by a kernel (safe inside captured graphs)

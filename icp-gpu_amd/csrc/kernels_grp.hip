@@ -603,6 +603,18 @@ int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const
   }
 }
 
+// wavefronts of a one-workgroup-per-tile launch of M items, against the GPU's SIMDs
+static int grp_waves_t(int G, int M) {
+  switch (G) {
+#define WV_CASE(g) case g: return (M + Cfg<g>::IPW - 1) / Cfg<g>::IPW * Cfg<g>::W;
+    WV_CASE(8) WV_CASE(9) WV_CASE(12) WV_CASE(13) WV_CASE(16) WV_CASE(18) WV_CASE(20) WV_CASE(24) WV_CASE(27)
+    WV_CASE(32)
+#undef WV_CASE
+    default: return 0;
+  }
+}
+bool grp_fills_simds(int G, int M) { return grp_waves_t(G, M) >= 4 * num_cus(); }
+
 bool grp_lanes_supported(int which, int G) {
   if (which == 0) return G == 8 || G == 12 || G == 13 || G == 16 || G == 20 || G == 24 || G == 32;
   return G == 9 || G == 12 || G == 16 || G == 18 || G == 24 || G == 27 || G == 32;

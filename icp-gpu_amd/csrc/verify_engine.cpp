@@ -37,24 +37,32 @@ static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
 
 // decode -> fork: [subgroup test on s2] || [Miller loop -> final exp on s] -> join ->
 // subgroup verdicts applied.  The subgroup test does not feed the pairing, so in the
-// lanes-per-item mode it overlaps both pairing kernels (which run at a raised wave
-// priority, so the test takes the issue slots they leave); the thread-per-item mode
-// joins before its final exponentiation, which reads the merged status.
+// lanes-per-item mode it overlaps the pairing kernels (which run at a raised wave
+// priority, so the test takes the issue slots they leave).  It starts with the Miller
+// loop, unless the Miller loop's one-workgroup-per-tile grid puts a wavefront on every
+// SIMD and the final exponentiation's does not: then it starts with the final
+// exponentiation, on the SIMDs that one leaves free (a wavefront sharing a SIMD with a
+// lone pairing wavefront slows it by a third).  The thread-per-item mode joins before
+// its final exponentiation, which reads the merged status.
+static int launch_subgroup_fork(const EngineArgs& a, bool fork, hipStream_t s) {
+  if (!fork) return launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, s);
+  if (hipEventRecord(a.fork, s) != hipSuccess || hipStreamWaitEvent(a.s2, a.fork, 0) != hipSuccess) return -1;
+  if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, a.s2)) return -1;
+  return hipEventRecord(a.join, a.s2) == hipSuccess ? 0 : -1;
+}
 int engine_compressed(const EngineArgs& a, hipStream_t s) {
   if (mark(a, 0, s)) return -1;
   if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 2, s)) return -1;
   if (launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
   if (mark(a, 1, s) || launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
   const bool fork = a.s2 && a.fork && a.join;
-  if (fork) {
-    if (hipEventRecord(a.fork, s) != hipSuccess || hipStreamWaitEvent(a.s2, a.fork, 0) != hipSuccess) return -1;
-    if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, a.s2)) return -1;
-    if (hipEventRecord(a.join, a.s2) != hipSuccess) return -1;
-  } else if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, s)) {
-    return -1;
-  }
-  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s)) return -1;
   const bool late = a.mode == 1;
+  const bool tiles_ml = !(a.queue && a.wpc_ml >= 0), tiles_fe = !(a.queue && a.wpc_fe >= 0);
+  const bool with_fe = late && fork && tiles_ml && tiles_fe && grp_fills_simds(a.g_ml, a.M) &&
+                       !grp_fills_simds(a.g_fe, a.M);
+  if (!with_fe && launch_subgroup_fork(a, fork, s)) return -1;
+  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s)) return -1;
+  if (with_fe && launch_subgroup_fork(a, fork, s)) return -1;
   if (fork && !late && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
   if (mark(a, 3, s) || final_exp(a, a.status, late ? nullptr : a.sub_status, s)) return -1;
   if (late) {

@@ -136,11 +136,11 @@ int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int
 int blsgpu_get_batched_engine(int M, int cfg[5]);
 /* out[0] graphs instantiated, out[1] graph launches */
 int blsgpu_batched_stats(int out[2]);
-/* Autotuner on a device-resident job of M items, timed with HIP events (best of 2).
+/* Autotuner on a device-resident job of M items, timed with HIP events (the median of three runs).
  * It times every final-exponentiation candidate alone, then every Miller-loop candidate
- * inside the whole kernel sequence with that final exponentiation (the subgroup test
- * runs concurrently with the pairing kernels, so their best configuration depends on
- * it).
+ * with each of the fastest final-exponentiation candidates inside the whole kernel
+ * sequence (the subgroup test runs concurrently with the pairing kernels, so their best
+ * configuration depends on it).
  * form 0 (batched): candidates are lanes per item x workgroups per CU; the fastest pair
  *   is recorded for M's size class (ceil(log2 M)); later batched calls and serving steps
  *   use the entry of the nearest class.
