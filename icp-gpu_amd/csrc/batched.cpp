@@ -341,18 +341,23 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
   // times one phase (1 final exponentiation, 2 the whole sequence) of configuration a: the
   // median of three runs after one untimed run (the whole sequence overlaps two streams,
   // and the best of a few runs can be a run where the concurrent kernel happened to
-  // interfere less than it does in steady state)
-  auto time_phase = [&](int which, double* ms) -> int {
+  // interfere less than it does in steady state).  A candidate whose first timed run is
+  // 25 % slower than `bound` (the best so far) is not run again.
+  auto time_phase = [&](int which, double* ms, double bound) -> int {
     float t[3];
+    int n = 0;
     for (int r = 0; r < 4; r++) {
       HIPCHK(hipEventRecord(e0, g.s));
       CHK(which == 1 ? bls::engine_final_exp(a, g.s) : bls::engine_compressed(a, g.s));
       HIPCHK(hipEventRecord(e1, g.s));
       HIPCHK(hipEventSynchronize(e1));
-      if (r > 0) HIPCHK(hipEventElapsedTime(&t[r - 1], e0, e1));
+      if (r == 0) continue;
+      HIPCHK(hipEventElapsedTime(&t[n], e0, e1));
+      n++;
+      if (r == 1 && t[0] > 1.25 * bound) break;
     }
-    std::sort(t, t + 3);
-    *ms = t[1];
+    std::sort(t, t + n);
+    *ms = t[n / 2];
     return 0;
   };
   const int gml[] = {8, 12, 13, 16, 20, 24, 32}, gfe[] = {9, 12, 16, 18, 24, 27, 32};
@@ -372,7 +377,9 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
       a.g_fe = G;
       a.wpc_fe = w;
       double t = 0;
-      CHK(time_phase(1, &t));
+      CHK(time_phase(1, &t, fe.empty() ? 1e30 : std::min_element(fe.begin(), fe.end(), [](const Cand& x, const Cand& y) {
+                                                   return x.ms < y.ms;
+                                                 })->ms));
       fe.push_back({t, G, w});
     }
   }
@@ -395,7 +402,7 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
         a.g_ml = G;
         a.wpc_ml = w;
         double t = 0;
-        CHK(time_phase(2, &t));
+        CHK(time_phase(2, &t, best_ml));
         if (t < best_ml) {
           best_ml = t;
           c.g_ml = G;

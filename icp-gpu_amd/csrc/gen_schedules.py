@@ -657,9 +657,18 @@ def emit_tables(G, ops):
     op_rows, round_rows, words = [], [], []
     for name, prounds, orounds in ops:
         op_rows.append("{%d, %d, %d}" % (len(round_rows), len(prounds), len(orounds)))
-        for na, nb, recs in prounds + orounds:
+        for k, (na, nb, recs) in enumerate(prounds + orounds):
             q = len(recs[0]) // 8
-            round_rows.append("{%d, %d, %d, %d, 0}" % (len(words) // 4, na, nb, q))
+            flags = 0
+            if k < len(prounds):
+                busy = [r for r in recs if r[0] != IDLE]
+                # an operand that is one slot with coefficient +1 on every busy lane is
+                # loaded as it is (slot values are valid multiplier inputs)
+                if na == 1 and all((r[1] >> 8) & 0xFF == 1 for r in busy):
+                    flags |= 1
+                if nb == 1 and all((r[B_AT] >> 8) & 0xFF == 1 for r in busy):
+                    flags |= 2
+            round_rows.append("{%d, %d, %d, %d, %d}" % (len(words) // 4, na, nb, q, flags))
             for rec in recs:
                 for k in range(0, len(rec), 2):
                     words.append(rec[k] | (rec[k + 1] << 16))
@@ -697,7 +706,9 @@ def main():
     lines.append("constexpr int MAX_OUT_Q = %d;" % MAX_OUT_Q)
     lines.append("constexpr uint16_t IDLE = 0x%x;" % IDLE)
     lines.append("struct OpRounds { uint16_t first; uint8_t np, no; };")
-    lines.append("struct Round { uint32_t desc; uint8_t na, nb, q, pad; };")
+    lines.append("// flags (product rounds): 1 = operand A is one slot with coefficient 1 on every")
+    lines.append("// busy lane, 2 = the same for B")
+    lines.append("struct Round { uint32_t desc; uint8_t na, nb, q, flags; };")
     lines.append("// the op and round tables sit in the constant address space, so that loads at")
     lines.append("// uniform indices are scalar loads")
     lines.append("typedef const OpRounds __attribute__((address_space(4))) COpRounds;")

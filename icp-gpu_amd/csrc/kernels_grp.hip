@@ -178,7 +178,7 @@ __device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* r
   u32 cur[REC_WORDS], nxt[REC_WORDS];
   if (nr > 0) load_record(cur, desc, rounds[first].desc, rounds[first].q, sub);
   for (int r = 0; r < nr; r++) {
-    const int na = rounds[first + r].na, nb = rounds[first + r].nb;
+    const int na = rounds[first + r].na, nb = rounds[first + r].nb, fl = rounds[first + r].flags;
     if (r + 1 < nr) load_record(nxt, desc, rounds[first + r + 1].desc, rounds[first + r + 1].q, sub);
     const u32 dst = rec_hw(cur, 0);
     // a wavefront none of whose lanes has a product or output in this round skips it
@@ -190,12 +190,23 @@ __device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* r
     for (int i = 0; i < NL; i++) acc[i] = 0;
     if (r < np) {
       if (work) {
-        accumulate<1, 8>(acc, cur, na, R);
-        const Fp a = carry64(acc);
+        Fp a, b;
+        if (fl & 1) {
+          const u32 t = rec_hw(cur, 1);
+          a = lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
+        } else {
+          accumulate<1, 8>(acc, cur, na, R);
+          a = carry64(acc);
 #pragma unroll
-        for (int i = 0; i < NL; i++) acc[i] = 0;
-        accumulate<sched::B_AT, 8>(acc, cur, nb, R);
-        const Fp b = carry64(acc);
+          for (int i = 0; i < NL; i++) acc[i] = 0;
+        }
+        if (fl & 2) {
+          const u32 t = rec_hw(cur, sched::B_AT);
+          b = lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
+        } else {
+          accumulate<sched::B_AT, 8>(acc, cur, nb, R);
+          b = carry64(acc);
+        }
         const Fp p = mul(a, b);
         if (active && dst != sched::IDLE) lds_st(T + (int)dst * SLOT, p);
       }

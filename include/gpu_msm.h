@@ -136,7 +136,8 @@ int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int
 int blsgpu_get_batched_engine(int M, int cfg[5]);
 /* out[0] graphs instantiated, out[1] graph launches */
 int blsgpu_batched_stats(int out[2]);
-/* Autotuner on a device-resident job of M items, timed with HIP events (the median of three runs).
+/* Autotuner on a device-resident job of M items, timed with HIP events (the median of three runs;
+ * a candidate 25 % slower than the best so far on its first run is not run again).
  * It times every final-exponentiation candidate alone, then every Miller-loop candidate
  * with each of the fastest final-exponentiation candidates inside the whole kernel
  * sequence (the subgroup test runs concurrently with the pairing kernels, so their best

@@ -84,6 +84,30 @@ def variants(progs):
     return [progs, GSCH.op_programs(cap=4)]
 
 
+def test_copy_flags_of_the_emitted_tables(variants):
+    """a round flagged as a plain copy of operand A (B) really holds one +1 term per busy
+    lane: the kernel then loads that slot without accumulating"""
+    path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
+    text = open(path).read()
+    for G in GSCH.GS:
+        ops = GSCH.build_rounds_v(G, variants)
+        rows = text.split("__constant__ const Round ROUNDS%d[" % G)[1].split("};")[0].split("= {", 1)[1]
+        flags = [int(x.strip("{} ").split(",")[4]) for x in rows.split("}, {")]
+        k = 0
+        for name, prounds, orounds, _, _ in ops:
+            for na, nb, recs in prounds:
+                busy = [r for r in recs if r[0] != GSCH.IDLE]
+                if flags[k] & 1:
+                    assert na == 1 and all(decode(r[1])[2] == 1 for r in busy), (G, name)
+                if flags[k] & 2:
+                    assert nb == 1 and all(decode(r[GSCH.B_AT])[2] == 1 for r in busy), (G, name)
+                k += 1
+            for _ in orounds:
+                assert flags[k] == 0
+                k += 1
+        assert k == len(flags)
+
+
 @pytest.mark.parametrize("G", GSCH.GS)
 def test_round_tables_execute_every_op(variants, G):
     """the tables the kernels use: per op the cheaper of the two programs"""
