@@ -115,6 +115,63 @@ enum DecodeStatus : int32_t {
   DEC_NOT_IN_SUBGROUP = -4,
   DEC_BAD_KEY = -5,  // set by the Miller kernel: key index outside the registry
 };
+
+// Jacobian + Jacobian (add-2007-bl), a = 0, with the doubling and identity cases.
+__device__ __forceinline__ Jac<Fp> jadd_jac(const Jac<Fp>& p, const Jac<Fp>& q) {
+  if (is_zero(p.z)) return q;
+  if (is_zero(q.z)) return p;
+  Fp z1z1 = sqr(p.z), z2z2 = sqr(q.z);
+  Fp u1 = mul(p.x, z2z2), u2 = mul(q.x, z1z1);
+  Fp s1 = mul(p.y, mul(q.z, z2z2)), s2 = mul(q.y, mul(p.z, z1z1));
+  Fp h = norm(sub(u2, u1));
+  Fp r = norm(dbl(sub(s2, s1)));
+  if (is_zero(h)) {
+    if (is_zero(r)) return jdbl(p);
+    return {fp_one(), fp_one(), fp_zero()};
+  }
+  Fp i = sqr(norm(dbl(h)));
+  Fp j = mul(h, i);
+  Fp v = mul(u1, i);
+  Jac<Fp> o;
+  o.x = norm(sub(sub(sub(sqr(r), j), v), v));
+  o.y = norm(sub(mul(r, sub(v, o.x)), dbl(mul(s1, j))));
+  o.z = mul(norm(sub(sub(sqr(carry(add(p.z, q.z))), z1z1), z2z2)), h);
+  return o;
+}
+__device__ __forceinline__ Jac<Fp> mul_absx_jac(const Jac<Fp>& q) {
+  Jac<Fp> acc = q;
+#pragma unroll 1
+  for (int i = 62; i >= 0; i--) {
+    acc = jdbl(acc);
+    if ((BLS_X >> i) & 1) acc = jadd_jac(acc, q);
+  }
+  return acc;
+}
+// word w0 + j of item i in a word-major (SoA) array of M items
+__device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < NL; j++) r.l[j] = base[(size_t)(w0 + j) * M + i];
+  return r;
+}
+// Subgroup status of decoded item i of M (points word-major: x in words 0..13, y in
+// 14..27): DEC_OK for items that did not decode (their status stands) and for points of
+// G1, DEC_NOT_IN_SUBGROUP otherwise.  [x^2]P is computed as [|x|]([|x|]P) without
+// leaving Jacobian coordinates, and compared with (beta x, -y) projectively:
+// X == beta x Z^2 and Y == -y Z^3 (the test of g1_in_subgroup).
+__device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32* status, int M, int i) {
+  if (status[i] != DEC_OK) return DEC_OK;
+  const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
+  Jac<Fp> p1 = {x, y, fp_one()};
+  Jac<Fp> a = mul_absx_jac(p1);
+  Jac<Fp> b = mul_absx_jac(a);
+  bool in = false;
+  if (!is_zero(b.z)) {
+    Fp z2 = sqr(b.z);
+    in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
+  }
+  return in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
+}
 // Decode 48 big-endian bytes given as 12 big-endian-packed 32-bit words (w[0] holds bytes
 // 0..3).  Semantics of ic_bls12_381 G1Affine::from_compressed: the reference GPU kernel
 // (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the flag handling, the

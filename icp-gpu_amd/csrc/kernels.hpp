@@ -23,17 +23,32 @@ int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines
 int launch_final_exp(const int32_t* fin, const int32_t* sig_status, int32_t* result, uint64_t* gt_out, int M,
                      int block, hipStream_t s);
 
+// The G1 subgroup test of a batch carried by a pairing launch: `per` items (one thread
+// each) in every workgroup past the launch's tiles, out[i] = subgroup status of item i of
+// the decoded points pts / status (per = 0: none).
+struct SubgroupRiders {
+  const int32_t* pts = nullptr;
+  const int32_t* status = nullptr;
+  int32_t* out = nullptr;
+  int per = 0;
+};
+
 // lanes-per-pairing kernels (kernels_grp.hip); G = lanes per item (Miller: 8, 12, 13, 16, 20,
 // 24, 32; final exponentiation: 9, 12, 16, 18, 24, 27, 32; 24 and 32 run two wavefronts per
 // workgroup).  Return -1 for an unsupported G or a launch error.
-// queue == nullptr: one workgroup per tile of its items.  Otherwise a persistent grid
-// (at most wg_per_cu workgroups per CU, 0 = the occupancy limit) pops tiles from the
-// device counter *queue, which must be zero at launch.
+// queue == nullptr: one workgroup per tile of its items, then ceil(M / riders.per)
+// workgroups of subgroup tests.  Otherwise a persistent grid (at most wg_per_cu
+// workgroups per CU, 0 = the occupancy limit) pops tiles from the device counter *queue,
+// which must be zero at launch; riders must then be empty.
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                       const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                      int* queue, int wg_per_cu, hipStream_t s);
+                      int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s);
+                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+// Items per subgroup-test workgroup that a one-workgroup-per-tile launch of M items at G
+// lanes per item (which: 0 Miller, 1 final exp) can carry on the SIMDs its tiles leave
+// without a wavefront; 0 if those SIMDs cannot hold the test of M items.
+int grp_rider_items(int which, int G, int M);
 // whether a lanes-per-item kernel (which: 0 Miller, 1 final exp) exists for G lanes
 bool grp_lanes_supported(int which, int G);
 // whether one workgroup per tile of M items at G lanes per item gives every SIMD a wavefront

@@ -12,12 +12,6 @@
 
 namespace bls {
 
-__device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
-  Fp r;
-#pragma unroll
-  for (int j = 0; j < NL; j++) r.l[j] = base[(size_t)(w0 + j) * M + i];
-  return r;
-}
 __device__ __forceinline__ void st_soa(i32* base, int w0, int M, int i, const Fp& v) {
 #pragma unroll
   for (int j = 0; j < NL; j++) base[(size_t)(w0 + j) * M + i] = v.l[j];
@@ -39,58 +33,12 @@ __global__ __launch_bounds__(64) void k_g1_decode(const uint8_t* __restrict__ in
   status[i] = st;
 }
 
-// Jacobian + Jacobian (add-2007-bl), a = 0, with the doubling and identity cases.
-__device__ __forceinline__ Jac<Fp> jadd(const Jac<Fp>& p, const Jac<Fp>& q) {
-  if (is_zero(p.z)) return q;
-  if (is_zero(q.z)) return p;
-  Fp z1z1 = sqr(p.z), z2z2 = sqr(q.z);
-  Fp u1 = mul(p.x, z2z2), u2 = mul(q.x, z1z1);
-  Fp s1 = mul(p.y, mul(q.z, z2z2)), s2 = mul(q.y, mul(p.z, z1z1));
-  Fp h = norm(sub(u2, u1));
-  Fp r = norm(dbl(sub(s2, s1)));
-  if (is_zero(h)) {
-    if (is_zero(r)) return jdbl(p);
-    return {fp_one(), fp_one(), fp_zero()};
-  }
-  Fp i = sqr(norm(dbl(h)));
-  Fp j = mul(h, i);
-  Fp v = mul(u1, i);
-  Jac<Fp> o;
-  o.x = norm(sub(sub(sub(sqr(r), j), v), v));
-  o.y = norm(sub(mul(r, sub(v, o.x)), dbl(mul(s1, j))));
-  o.z = mul(norm(sub(sub(sqr(carry(add(p.z, q.z))), z1z1), z2z2)), h);
-  return o;
-}
-__device__ __forceinline__ Jac<Fp> mul_absx_jac(const Jac<Fp>& q) {
-  Jac<Fp> acc = q;
-  for (int i = 62; i >= 0; i--) {
-    acc = jdbl(acc);
-    if ((BLS_X >> i) & 1) acc = jadd(acc, q);
-  }
-  return acc;
-}
-
-// Points whose decode status is OK get sub_status 0 (in G1) or DEC_NOT_IN_SUBGROUP.
-// [x^2]P is computed as [|x|]([|x|]P) without leaving Jacobian coordinates, and compared
-// with (beta x, -y) projectively: X == beta x Z^2 and Y == -y Z^3.
+// one thread per item (g1_subgroup_status)
 __global__ __launch_bounds__(64) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
                                                     i32* __restrict__ sub_status, int M) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
-  if (status[i] != DEC_OK) {
-    sub_status[i] = DEC_OK;
-    return;
-  }
-  const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
-  Jac<Fp> p1 = {x, y, fp_one()};
-  Jac<Fp> a = mul_absx_jac(p1);
-  Jac<Fp> b = mul_absx_jac(a);
-  bool in = false;
-  if (!is_zero(b.z)) {
-    Fp z2 = sqr(b.z);
-    in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
-  }
-  sub_status[i] = in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
+  sub_status[i] = g1_subgroup_status(pts, status, M, i);
 }
 
 // status := sub_status where status is OK (for entry points that report decoding alone)

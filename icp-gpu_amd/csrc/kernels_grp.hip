@@ -370,15 +370,38 @@ __device__ __forceinline__ int next_tile(int* queue, int round) {
   return t;
 }
 
+// A workgroup past a launch's tiles: thread t tests item k * per + t of the batch for
+// membership of G1 (the subgroup test needs only the decoded signature, and no pairing
+// value needs it).  These workgroups run at the default wave priority on the SIMDs the
+// tiles leave free.
+__device__ __forceinline__ void subgroup_rider(const i32* pts, const i32* status, i32* out, int M, int per, int k) {
+  const int t = (int)threadIdx.x;
+  const int i = k * per + t;
+  if (t < per && i < M) out[i] = g1_subgroup_status(pts, status, M, i);
+}
+
+// Lane counts whose one-workgroup-per-tile grid leaves SIMDs free at batch sizes near a
+// full GPU (8 / 9 / 12 lanes: 8, 7, 5 items per wavefront) have a second instantiation
+// of each kernel that carries the subgroup test (RIDE); its register budget is that of
+// one wavefront per SIMD, which the subgroup test's double-and-add uses.
 template <int G>
-__global__ __launch_bounds__(Cfg<G>::THREADS, Cfg<G>::W) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
+constexpr bool rides(int which) { return which == 0 ? (G == 8 || G == 12) : (G == 9 || G == 12); }
+
+template <int G, bool RIDE>
+__global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : Cfg<G>::W) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                                    const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                                    const int32_t* __restrict__ key_idx, int nkeys,
                                                    const i32* __restrict__ pk_soa, i32* __restrict__ status,
-                                                   i32* __restrict__ fout, int M, int* queue) {
+                                                   i32* __restrict__ fout, int M, int* queue, SubgroupRiders riders) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * ML_SLOTS * SLOT;
+  if constexpr (RIDE) {
+    if (!queue && (int)blockIdx.x >= ntiles) {
+      subgroup_rider(riders.pts, riders.status, riders.out, M, riders.per, (int)blockIdx.x - ntiles);
+      return;
+    }
+  }
   // raised wave priority: a concurrent subgroup-test wave on the same SIMD issues in the
   // cycles this wave leaves idle instead of halving its issue rate
   __builtin_amdgcn_s_setprio(2);
@@ -524,13 +547,19 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
   __syncthreads();  // the next tile reuses this tile's LDS
 }
 
-template <int G>
-__global__ __launch_bounds__(Cfg<G>::THREADS, Cfg<G>::W) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
+template <int G, bool RIDE>
+__global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : Cfg<G>::W) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
                                                       const i32* __restrict__ sub_status, int32_t* __restrict__ result,
-                                                      u64* __restrict__ gt_out, int M, int* queue) {
+                                                      u64* __restrict__ gt_out, int M, int* queue, SubgroupRiders riders) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   const int K = IPW * FE_SLOTS * SLOT;
+  if constexpr (RIDE) {
+    if (!queue && (int)blockIdx.x >= ntiles) {
+      subgroup_rider(riders.pts, riders.status, riders.out, M, riders.per, (int)blockIdx.x - ntiles);
+      return;
+    }
+  }
   __builtin_amdgcn_s_setprio(2);
   load_consts(K);
   for (int round = 0;; round++) {
@@ -554,13 +583,13 @@ static int num_cus() {
 template <int G>
 static int ml_occ() {
   static int occ = 0;
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G>, Cfg<G>::THREADS, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G, false>, Cfg<G>::THREADS, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
 }
 template <int G>
 static int fe_occ() {
   static int occ = 0;
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G>, Cfg<G>::THREADS, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G, false>, Cfg<G>::THREADS, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
 }
 // Grid of a queue-fed launch: at most wg_per_cu workgroups per CU (0 = as many as
@@ -571,43 +600,75 @@ static int queue_grid(int occ, int ntiles, int wg_per_cu) {
   return (int)(g < ntiles ? g : ntiles);
 }
 
+// grid of a launch: the persistent grid with a queue, else one workgroup per tile plus
+// the subgroup-test workgroups (riders with per <= threads of a workgroup, no queue)
+static int launch_grid(int occ, int ntiles, int threads, int* queue, int wg_per_cu, const SubgroupRiders& r, int M,
+                       int* grid) {
+  if (queue) {
+    if (r.per) return -1;
+    *grid = queue_grid(occ, ntiles, wg_per_cu);
+    return 0;
+  }
+  if (r.per < 0 || r.per > threads || (r.per && (!r.pts || !r.status || !r.out))) return -1;
+  *grid = ntiles + (r.per ? (M + r.per - 1) / r.per : 0);
+  return 0;
+}
+
 template <int G>
 static int launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                     int* queue, int wg_per_cu, hipStream_t s) {
+                     int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
-  const int grid = queue ? queue_grid(ml_occ<G>(), ntiles, wg_per_cu) : ntiles;
-  hipLaunchKernelGGL(k_miller_grp<G>, dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen, keys, key_idx,
-                     nkeys, pk_soa, status, fout, M, queue);
+  int grid = 0;
+  if (riders.per && !rides<G>(0)) return -1;
+  if (launch_grid(queue ? ml_occ<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  if constexpr (rides<G>(0)) {
+    if (riders.per) {
+      hipLaunchKernelGGL((k_miller_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen,
+                         keys, key_idx, nkeys, pk_soa, status, fout, M, queue, riders);
+      return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+  }
+  hipLaunchKernelGGL((k_miller_grp<G, false>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen, keys,
+                     key_idx, nkeys, pk_soa, status, fout, M, queue, riders);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int G>
 static int launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                     uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
+                     uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
-  const int grid = queue ? queue_grid(fe_occ<G>(), ntiles, wg_per_cu) : ntiles;
-  hipLaunchKernelGGL(k_final_exp_grp<G>, dim3(grid), dim3(Cfg<G>::THREADS), final_exp_lds_bytes<G>(), s, fin, status, sub_status,
-                     result, gt, M, queue);
+  int grid = 0;
+  if (riders.per && !rides<G>(1)) return -1;
+  if (launch_grid(queue ? fe_occ<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  if constexpr (rides<G>(1)) {
+    if (riders.per) {
+      hipLaunchKernelGGL((k_final_exp_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), final_exp_lds_bytes<G>(), s, fin,
+                         status, sub_status, result, gt, M, queue, riders);
+      return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+  }
+  hipLaunchKernelGGL((k_final_exp_grp<G, false>), dim3(grid), dim3(Cfg<G>::THREADS), final_exp_lds_bytes<G>(), s, fin, status,
+                     sub_status, result, gt, M, queue, riders);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                       const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                      int* queue, int wg_per_cu, hipStream_t s) {
+                      int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   switch (G) {
 #define ML_CASE(g) \
-  case g: return launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, s);
+  case g: return launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, riders, s);
     ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(20) ML_CASE(24) ML_CASE(32)
 #undef ML_CASE
     default: return -1;
   }
 }
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, int* queue, int wg_per_cu, hipStream_t s) {
+                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   switch (G) {
-#define FE_CASE(g) case g: return launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, s);
+#define FE_CASE(g) case g: return launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, riders, s);
     FE_CASE(9) FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(24) FE_CASE(27) FE_CASE(32)
 #undef FE_CASE
     default: return -1;
@@ -625,6 +686,35 @@ static int grp_waves_t(int G, int M) {
   }
 }
 bool grp_fills_simds(int G, int M) { return grp_waves_t(G, M) >= 4 * num_cus(); }
+
+static int grp_threads(int G) {
+  switch (G) {
+#define TH_CASE(g) case g: return Cfg<g>::THREADS;
+    TH_CASE(8) TH_CASE(9) TH_CASE(12) TH_CASE(13) TH_CASE(16) TH_CASE(18) TH_CASE(20) TH_CASE(24) TH_CASE(27)
+    TH_CASE(32)
+#undef TH_CASE
+    default: return 0;
+  }
+}
+// Full workgroups of riders (one item per thread), all on SIMDs that the tiles leave
+// without a wavefront.  A rider wavefront's run time (two 64-step double-and-add chains,
+// about half of a pairing wavefront's) does not lengthen the launch.
+static bool grp_rides(int which, int G) {
+  switch (G) {
+#define RD_CASE(g) case g: return rides<g>(which);
+    RD_CASE(8) RD_CASE(9) RD_CASE(12) RD_CASE(13) RD_CASE(16) RD_CASE(18) RD_CASE(20) RD_CASE(24) RD_CASE(27)
+    RD_CASE(32)
+#undef RD_CASE
+    default: return false;
+  }
+}
+int grp_rider_items(int which, int G, int M) {
+  if (M <= 0 || !grp_lanes_supported(which, G) || !grp_rides(which, G)) return 0;
+  const int threads = grp_threads(G);
+  const long free_waves = 4L * num_cus() - grp_waves_t(G, M);
+  const long rider_waves = (long)(M + threads - 1) / threads * (threads / 64);
+  return free_waves > 0 && rider_waves <= free_waves ? threads : 0;
+}
 
 bool grp_lanes_supported(int which, int G) {
   if (which == 0) return G == 8 || G == 12 || G == 13 || G == 16 || G == 20 || G == 24 || G == 32;

@@ -6,20 +6,21 @@
 namespace bls {
 
 static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, const int32_t* key_idx,
-                  const int32_t* pk_soa, int32_t* status, hipStream_t s) {
+                  const int32_t* pk_soa, int32_t* status, hipStream_t s, const SubgroupRiders& riders = {}) {
   if (a.mode == 0)
     return launch_miller(a.sig_pts, a.nhm_pts, gen, key_idx ? keys : pk_soa, key_idx, a.nkeys, status, a.fbuf, a.M,
                          a.block, s);
   return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_soa, status, a.fbuf, a.M,
-                           a.queue && a.wpc_ml >= 0 ? a.queue : nullptr, a.wpc_ml, s);
+                           a.queue && a.wpc_ml >= 0 ? a.queue : nullptr, a.wpc_ml, riders, s);
 }
-static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* sub_status, hipStream_t s) {
+static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* sub_status, hipStream_t s,
+                     const SubgroupRiders& riders = {}) {
   if (a.mode == 0) {
     if (sub_status && launch_merge_status(const_cast<int32_t*>(status), sub_status, a.M, s)) return -1;
     return launch_final_exp(a.fbuf, status, a.results, a.gt, a.M, a.block, s);
   }
   return launch_final_exp_grp(a.g_fe, a.fbuf, status, sub_status, a.results, a.gt, a.M,
-                              a.queue && a.wpc_fe >= 0 ? a.queue + 1 : nullptr, a.wpc_fe, s);
+                              a.queue && a.wpc_fe >= 0 ? a.queue + 1 : nullptr, a.wpc_fe, riders, s);
 }
 
 int engine_legacy(const EngineArgs& a, hipStream_t s) {
@@ -35,7 +36,7 @@ static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
   return a.ev ? (hipEventRecord(a.ev[k], s) == hipSuccess ? 0 : -1) : 0;
 }
 
-// decode -> fork: [subgroup test on s2] || [Miller loop -> final exp on s] -> join ->
+// Otherwise: decode -> fork: [subgroup test on s2] || [Miller loop -> final exp on s] -> join ->
 // subgroup verdicts applied.  The subgroup test does not feed the pairing, so in the
 // lanes-per-item mode it overlaps the pairing kernels (which run at a raised wave
 // priority, so the test takes the issue slots they leave).  It starts with the Miller
@@ -50,14 +51,33 @@ static int launch_subgroup_fork(const EngineArgs& a, bool fork, hipStream_t s) {
   if (launch_g1_subgroup(a.sig_pts, a.status, a.sub_status, a.M, a.s2)) return -1;
   return hipEventRecord(a.join, a.s2) == hipSuccess ? 0 : -1;
 }
+// decode -> Miller loop -> final exp, the subgroup test riding on the first pairing
+// launch whose one-workgroup-per-tile grid leaves SIMDs enough for it: the Miller loop
+// (the final exponentiation then reads the subgroup status), else the final
+// exponentiation (its verdicts are corrected afterwards).
+static int engine_riders(const EngineArgs& a, const SubgroupRiders& rml, const SubgroupRiders& rfe, hipStream_t s) {
+  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s, rml)) return -1;
+  if (mark(a, 3, s) || final_exp(a, a.status, rml.per ? a.sub_status : nullptr, s, rfe)) return -1;
+  if (rfe.per && launch_apply_subgroup(a.results, a.gt, a.status, a.sub_status, a.M, s)) return -1;
+  return mark(a, 4, s);
+}
+
 int engine_compressed(const EngineArgs& a, hipStream_t s) {
   if (mark(a, 0, s)) return -1;
   if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 2, s)) return -1;
   if (launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
   if (mark(a, 1, s) || launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
+  const bool tiles_ml = !(a.queue && a.wpc_ml >= 0), tiles_fe = !(a.queue && a.wpc_fe >= 0);
+  if (a.mode == 1) {
+    SubgroupRiders rml, rfe;
+    const int per_ml = tiles_ml ? grp_rider_items(0, a.g_ml, a.M) : 0;
+    const int per_fe = !per_ml && tiles_fe ? grp_rider_items(1, a.g_fe, a.M) : 0;
+    if (per_ml) rml = {a.sig_pts, a.status, a.sub_status, per_ml};
+    if (per_fe) rfe = {a.sig_pts, a.status, a.sub_status, per_fe};
+    if (per_ml || per_fe) return engine_riders(a, rml, rfe, s);
+  }
   const bool fork = a.s2 && a.fork && a.join;
   const bool late = a.mode == 1;
-  const bool tiles_ml = !(a.queue && a.wpc_ml >= 0), tiles_fe = !(a.queue && a.wpc_fe >= 0);
   const bool with_fe = late && fork && tiles_ml && tiles_fe && grp_fills_simds(a.g_ml, a.M) &&
                        !grp_fills_simds(a.g_fe, a.M);
   if (!with_fe && launch_subgroup_fork(a, fork, s)) return -1;

@@ -77,3 +77,30 @@ def test_bad_key_index_is_reported():
 def test_restore_default_engine():
     blsgpu.set_engine(1, 12, 16)
     assert blsgpu.get_engine() == (1, 12, 16)
+
+
+@pytest.mark.parametrize("cfg", [(1, 12, 16), (1, 16, 12), (1, 16, 16)], ids=lambda c: "mode%d_ml%d_fe%d" % c)
+def test_subgroup_test_placements(cfg):
+    """The subgroup test carried by the Miller launch (12 lanes), by the final
+    exponentiation launch (12 lanes) and on the second stream (16/16): off-subgroup
+    signatures at the edges of the 64-item rider workgroups, among undecodable and
+    wrong-message items, over a batch of five rider workgroups."""
+    print("case: subgroup test placement", cfg, flush=True)
+    n = 300
+    sks, pks, _ = O.make_keys(43, 5)
+    key_idx = np.array([(i * 3) % 5 for i in range(n)], np.int32)
+    inv = {0: "non_subgroup", 63: "non_subgroup", 64: "non_subgroup", 100: "wrong_msg", 130: "non_subgroup",
+           200: "no_sqrt", 255: "non_subgroup", 256: "wrong_msg", 299: "non_subgroup"}
+    comp, _, nhm, _ = O.make_batch(43, n, sks, key_idx, inv)
+    want = []
+    for i in range(n):
+        rc, _ = O.g1_decompress(comp[i])
+        want.append(rc if rc != 0 else (0 if inv.get(i) == "wrong_msg" else 1))
+    want = np.array(want)
+    assert (want == -4).sum() == 6 and (want == 1).sum() == n - 9
+    blsgpu.set_engine(*cfg)
+    blsgpu.load().blsgpu_reset_keys()
+    hs = np.array([blsgpu.register_key_affine(pk) for pk in pks], np.int32)
+    res, gt = blsgpu.verify_compressed(comp, nhm, hs[key_idx], want_gt=True)
+    assert (res == want).all(), np.nonzero(res != want)
+    assert not gt[want < 0].any()
