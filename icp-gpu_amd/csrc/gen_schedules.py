@@ -679,6 +679,9 @@ def emit_tables(G, ops):
            "__device__ __align__(16) const uint32_t DESC%d[%d] = {%s};" % (
                G, len(words), ", ".join("0x%x" % w for w in words)),
            "template <> struct Tab<%d> {" % G,
+           "  // the same tables as compile-time constants (round shapes of the ops run as straight-line code)",
+           "  static constexpr OpRounds OPS_C[NUM_OPS] = {%s};" % ", ".join(op_rows),
+           "  static constexpr Round ROUNDS_C[%d] = {%s};" % (len(round_rows), ", ".join(round_rows)),
            "  static __device__ __forceinline__ const COpRounds* ops() { return (const COpRounds*)OPS%d; }" % G,
            "  static __device__ __forceinline__ const CRound* rounds() { return (const CRound*)ROUNDS%d; }" % G,
            "  static __device__ __forceinline__ const uint4* desc() { return reinterpret_cast<const uint4*>(DESC%d); }"

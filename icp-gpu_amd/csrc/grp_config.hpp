@@ -1,0 +1,52 @@
+// Configurations of the lanes-per-item kernels (kernels_grp.hip), shared by the kernel
+// translation units (one per lane count G, built with -DGRP_LANES=G) and the host-side
+// dispatch (grp_dispatch.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace bls {
+
+// Lane counts with a Miller-loop kernel and with a final-exponentiation kernel.
+#define BLS_GRP_ML_LANES(X) X(8) X(12) X(13) X(16) X(20) X(24) X(32)
+#define BLS_GRP_FE_LANES(X) X(9) X(12) X(16) X(18) X(24) X(27) X(32)
+#define BLS_GRP_ALL_LANES(X) X(8) X(9) X(12) X(13) X(16) X(18) X(20) X(24) X(27) X(32)
+
+// Layout of a lanes-per-item configuration G: W wavefronts per workgroup, L = G / W lanes
+// of each wavefront per item, IPW = 64 / L items per workgroup.  Lane s (0 <= s < G) of
+// an item sits in wavefront s / L at position item * L + s % L.
+template <int G>
+struct Cfg {
+  static constexpr int W = (G == 24 || G == 32) ? 2 : 1;
+  static constexpr int L = G / W;
+  static constexpr int IPW = 64 / L;
+  static constexpr int THREADS = 64 * W;
+  static_assert(L * W == G, "lanes per item");
+};
+
+// Lane counts whose one-workgroup-per-tile grid leaves SIMDs free at batch sizes near a
+// full GPU (8 / 9 / 12 lanes: 8, 7, 5 items per wavefront) have a second instantiation
+// of each kernel that carries the subgroup test (RIDE); its register budget is that of
+// one wavefront per SIMD, which the subgroup test's double-and-add uses.
+template <int G>
+constexpr bool rides(int which) { return which == 0 ? (G == 8 || G == 12) : (G == 9 || G == 12); }
+
+// Per-G launchers and occupancy queries, instantiated in the translation unit of G.
+template <int G>
+int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
+                  const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                  int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+template <int G>
+int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
+                  int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+template <int G>
+int grp_occ_ml();
+template <int G>
+int grp_occ_fe();
+
+// number of compute units of the current device (queried once)
+int grp_num_cus();
+
+}  // namespace bls

@@ -67,6 +67,14 @@ int launch_apply_subgroup(int32_t* result, uint64_t* gt, const int32_t* status, 
   hipLaunchKernelGGL(k_apply_subgroup, dim3((M + 255) / 256), dim3(256), 0, s, result, gt, status, sub_status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+__global__ void k_zero_words(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+int launch_zero_words(int* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_words, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s) {
   hipLaunchKernelGGL(k_g1_decode, dim3((M + 63) / 64), dim3(64), 0, s, in, pts, status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -2,38 +2,39 @@
 //
 // Each item owns a block of Fp "slots" in LDS (SLOT = 14 words, one per limb).  Every
 // extension-field operation is a bilinear schedule (schedules_gen.hpp): the group's
-// lanes split its independent Fp products, then its output combinations.  The only
-// multiplier body in the kernel is the one inside run_op(), so the code stays small
-// enough for the instruction cache and needs no calls or scratch.
+// lanes split its independent Fp products, then its output combinations.  Most ops run
+// through one round interpreter (run_op, a single multiplier body); the hot ones (the
+// Miller loop's line ops and square, the cyclotomic square and product of the final
+// exponentiation) run as straight-line code from the same tables taken as compile-time
+// constants (run_op_c), each emitted once.
 //
 // A workgroup is W wavefronts holding 64 / L items, L = G / W lanes of each wavefront
-// per item (Cfg below): one wavefront for G <= 27; two for G = 24 and 32, so that two
-// wavefronts share each SIMD when the items exactly fill the GPU at one workgroup per
-// tile (a lone wavefront issues at most one instruction per ~6 cycles, two together
-// one per ~4.9).  Lanes past the last full group idle but take part in the barriers.  The Miller value passes between the
-// two kernels item-major (168 consecutive words per item), so the G lanes of an item
-// read and write whole cache lines whichever XCD runs the workgroup.
+// per item (Cfg in grp_config.hpp): one wavefront for G <= 27; two for G = 24 and 32, so
+// that two wavefronts share each SIMD when the items exactly fill the GPU at one
+// workgroup per tile (a lone wavefront issues at most one instruction per ~6 cycles,
+// two together one per ~4.9).  Lanes past the last full group idle but take part in the
+// barriers.  The Miller value passes between the two kernels item-major (168
+// consecutive words per item), so the G lanes of an item read and write whole cache
+// lines whichever XCD runs the workgroup.
+//
+// This file is compiled once per lane count (-DGRP_LANES=G); the launchers of each G are
+// instantiated at the end, the dispatch over G is in grp_dispatch.cpp.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "curve.hpp"
+#include "grp_config.hpp"
 #include "kernels.hpp"
 #include "schedules_gen.hpp"
+
+#ifndef GRP_LANES
+#error "kernels_grp.hip is compiled once per lane count: -DGRP_LANES=<G>"
+#endif
 
 namespace bls {
 
 constexpr int SLOT = 14;
-
-// Layout of a lanes-per-item configuration G: W wavefronts per workgroup, L = G / W lanes
-// of each wavefront per item, IPW = 64 / L items per workgroup.  Lane s (0 <= s < G) of
-// an item sits in wavefront s / L at position item * L + s % L.
-template <int G>
-struct Cfg {
-  static constexpr int W = (G == 24 || G == 32) ? 2 : 1;
-  static constexpr int L = G / W;
-  static constexpr int IPW = 64 / L;
-  static constexpr int THREADS = 64 * W;
-  static_assert(L * W == G, "lanes per item");
-};
 
 // All LDS of these kernels is one dynamic array; device functions address it by word
 // offset so that they stay in the LDS address space across calls.
@@ -84,17 +85,23 @@ struct Regions {
 constexpr int REC_WORDS = 4 * sched::MAX_OUT_Q;
 static_assert(sched::PROD_Q <= sched::MAX_OUT_Q, "record buffer");
 
-__device__ __forceinline__ u32 rec_hw(const u32 (&rec)[REC_WORDS], int k) {
+template <int RW>
+__device__ __forceinline__ u32 rec_hw(const u32 (&rec)[RW], int k) {
   return (k & 1) ? (rec[k >> 1] >> 16) : (rec[k >> 1] & 0xffffu);
 }
+// the record tables, addressed as global memory (a generic pointer would make the record
+// loads flat loads, which the LDS wait counter also counts)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) GRec;
+
 // this lane's record of a round: q 16-byte units at desc + sub * q (q is uniform)
-__device__ __forceinline__ void load_record(u32 (&rec)[REC_WORDS], const uint4* __restrict__ desc, u32 at, int q,
+__device__ __forceinline__ void load_record(u32 (&rec)[REC_WORDS], GRec* __restrict__ desc, u32 at, int q,
                                             int sub) {
-  const uint4* p = desc + at + (u32)sub * q;
+  GRec* p = desc + at + (u32)sub * q;
 #pragma unroll
   for (int u = 0; u < sched::MAX_OUT_Q; u++) {
     if (u < q) {
-      const uint4 v = p[u];
+      const u32x4 v = p[u];
       rec[4 * u] = v.x;
       rec[4 * u + 1] = v.y;
       rec[4 * u + 2] = v.z;
@@ -160,10 +167,10 @@ __device__ __forceinline__ P* uniform_ptr(P* p) {
 // lane of the wavefront runs the same instruction stream whatever its record holds.
 // D may alias A (every op used in place reads no other input component in its
 // outputs).  Not inlined: one copy of the multiplier serves every call site.
-// One instantiation per wavefronts-per-workgroup count, so that the two-wavefront kernels'
-// register budget (two waves per SIMD) applies to their copy of the interpreter.
-template <int WAVES>
-__device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* rounds, const uint4* __restrict__ desc,
+// One instantiation per register budget of the calling kernels (WPE waves per SIMD), so
+// that each copy is compiled within its callers' budget.
+template <int WPE>
+__device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* rounds, GRec* __restrict__ desc,
                                          int op, int sub, bool active, int A, int B, int D, int T, int K) {
   // the table pointers, the op and the round shapes are the same on every lane: make
   // that visible to the compiler so that they live in scalar registers, the tables are
@@ -223,10 +230,123 @@ __device__ __noinline__ void run_op_impl(sched::COpRounds* ops, sched::CRound* r
   }
   __syncthreads();
 }
-template <int G>
+// WPE: the waves per SIMD that the calling kernel is compiled for (its register budget)
+template <int G, int WPE>
 __device__ __forceinline__ void run_op(int op, int sub, bool active, int A, int B, int D, int T, int K) {
-  run_op_impl<Cfg<G>::W>(sched::Tab<G>::ops(), sched::Tab<G>::rounds(), sched::Tab<G>::desc(), op, sub, active, A, B, D,
+  run_op_impl<WPE>(sched::Tab<G>::ops(), sched::Tab<G>::rounds(), (GRec*)sched::Tab<G>::desc(), op, sub, active, A, B, D,
                          T, K);
+}
+
+// ---- ops with compile-time round shapes ----
+// The hot ops (the cyclotomic square and the Fp12 product of the final exponentiation,
+// the line ops and the square of the Miller loop) run as straight-line code built from
+// the same tables taken as compile-time constants (Tab<G>::OPS_C / ROUNDS_C): term
+// counts, copy flags and record sizes are immediates, so a round has no scalar
+// bookkeeping, no per-term branches, no accumulator clearing (the first term sets it)
+// and no record copies.  The records themselves (which slots, which coefficients) are
+// still read per lane from DESC.
+
+// A uniform 1 the compiler cannot see through: code guarded by it forms basic blocks
+// of its own, which the scheduler does not interleave (one block per group of terms
+// keeps their LDS loads and arithmetic together, as in the interpreter).
+__device__ __forceinline__ int opaque_one() {
+  int v;
+  asm volatile("s_mov_b32 %0, 1" : "=s"(v));
+  return v;
+}
+
+// the sum of the N >= 1 terms at halfwords FIRST.. of the record, carried
+template <int FIRST, int N, int RW>
+__device__ __forceinline__ Fp sum_terms(const u32 (&rec)[RW], const Regions& R) {
+  static_assert(N >= 1 && FIRST + N <= 2 * RW, "terms of a busy round");
+  i64 acc[NL];
+#pragma unroll
+  for (int c = 0; c < N; c += 4) {
+    if (c == 0 || opaque_one()) {
+      Fp x[4];
+      i32 k[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (c + j < N) {
+          const u32 t = rec_hw(rec, FIRST + c + j);
+          x[j] = lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
+          k[j] = (i32)(int8_t)(t >> 8);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (c + j < N) {
+#pragma unroll
+          for (int i = 0; i < NL; i++) {
+            const i64 v = (i64)x[j].l[i] * k[j];
+            acc[i] = c + j == 0 ? v : acc[i] + v;
+          }
+        }
+      }
+    }
+  }
+  return carry64(acc);
+}
+template <int AT, int RW>
+__device__ __forceinline__ Fp load_term(const u32 (&rec)[RW], const Regions& R) {
+  const u32 t = rec_hw(rec, AT);
+  return lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
+}
+
+template <int G, int OP, int RI>
+__device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, bool active, const Regions& R,
+                                             int D) {
+  constexpr sched::OpRounds o = sched::Tab<G>::OPS_C[OP];
+  constexpr sched::Round rd = sched::Tab<G>::ROUNDS_C[o.first + RI];
+  constexpr int Q = rd.q, RW = 4 * Q;
+  u32 rec[RW];
+  GRec* p = desc + rd.desc + (u32)sub * Q;
+#pragma unroll
+  for (int u = 0; u < Q; u++) {
+    const u32x4 v = p[u];
+    rec[4 * u] = v.x;
+    rec[4 * u + 1] = v.y;
+    rec[4 * u + 2] = v.z;
+    rec[4 * u + 3] = v.w;
+  }
+  const u32 dst = rec[0] & 0xffffu;
+  // a wavefront none of whose lanes has a product or output in this round skips it
+  const bool work = __ballot(dst != sched::IDLE) != 0;
+  if constexpr (RI < o.np) {
+    if (work) {
+      Fp a, b;
+      if constexpr (rd.flags & 1) a = load_term<1>(rec, R);
+      else a = sum_terms<1, rd.na>(rec, R);
+      if constexpr (rd.flags & 2) b = load_term<sched::B_AT>(rec, R);
+      else b = sum_terms<sched::B_AT, rd.nb>(rec, R);
+      const Fp pr = mul(a, b);
+      if (active && dst != sched::IDLE) lds_st(R.t + (int)dst * SLOT, pr);
+    }
+    if constexpr (RI + 1 == o.np) __syncthreads();
+  } else {
+    if (work) {
+      const Fp v = fold_top(sum_terms<1, rd.na>(rec, R));
+      if (active && dst != sched::IDLE) lds_st(D + (int)dst * SLOT, v);
+    }
+  }
+}
+template <int G, int OP, int... RI>
+__device__ __forceinline__ void static_rounds(std::integer_sequence<int, RI...>, GRec* __restrict__ desc, int sub,
+                                              bool active, const Regions& R, int D) {
+  (static_round<G, OP, RI>(desc, sub, active, R, D), ...);
+}
+// run_op for an op known at compile time (same contract as run_op).  Inlined: its
+// callers hold it at one call site each inside a function entered rarely (the Miller
+// loop's kernel, the final exponentiation's cyclotomic power), as a call of its own
+// would save and restore the callee-saved registers it uses on every op.
+template <int G, int OP>
+__device__ __forceinline__ void run_op_c(int sub, bool active, int A, int B, int D, int T, int K) {
+  constexpr sched::OpRounds o = sched::Tab<G>::OPS_C[OP];
+  GRec* desc = uniform_ptr((GRec*)sched::Tab<G>::desc());
+  K = __builtin_amdgcn_readfirstlane(K);
+  const Regions R{A, B, T, K};
+  static_rounds<G, OP>(std::make_integer_sequence<int, o.np + o.no>{}, desc, sub, active, R, D);
+  __syncthreads();
 }
 
 __device__ __forceinline__ void load_consts(int K) {
@@ -247,7 +367,7 @@ constexpr size_t miller_lds_bytes() {
   return (size_t)(Cfg<G>::IPW * ML_SLOTS + sched::NUM_CONSTS) * SLOT * 4;
 }
 
-template <int G>
+template <int G, int WPE>
 __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                             const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                             const int32_t* __restrict__ key_idx, int nkeys,
@@ -318,36 +438,46 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
     // rides along with f * LL.  Region B of both ops is the line block L1 | L2 | P | S.
     load_lines(0);
     __syncthreads();
-    run_op<G>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);
-    auto mstep = [&](bool last) {
+    run_op<G, WPE>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);
+    // for b = 61 .. 0: a line step, a second one if bit b of |x| is set, the square of f;
+    // then the last line step.  One loop, so that each op's straight-line code is
+    // emitted once.
+    bool second = false;
+    for (int b = 61;;) {
+      const bool last = b < 0;
       if (!last) load_lines(ci + 1);  // L1 | L2 are free once S holds their scaled form
-      run_op<G>(sched::OP_LL, sub, act, 0, L1, S1, T, K);
-      run_op<G>(last ? sched::OP_FL : sched::OP_FL_SCALE, sub, act, F, L1, F, T, K);
+      run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
+      if (last) {
+        run_op<G, WPE>(sched::OP_FL, sub, act, F, L1, F, T, K);
+        break;
+      }
+      run_op_c<G, sched::OP_FL_SCALE>(sub, act, F, L1, F, T, K);
       ci++;
-    };
-    for (int b = 61; b >= 0; b--) {
-      mstep(false);
-      if (miller_bit(b)) mstep(false);
-      run_op<G>(sched::OP_SQR, sub, act, F, 0, F, T, K);
+      if (!second && miller_bit(b)) {
+        second = true;
+        continue;
+      }
+      second = false;
+      run_op_c<G, sched::OP_SQR>(sub, act, F, 0, F, T, K);
+      b--;
     }
-    mstep(true);
   } else {
     // a tile with an identity signature: its pair-1 lines are skipped item by item
     auto step = [&]() {
       load_lines(ci++);
       __syncthreads();
-      run_op<G>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);  // S1 | S2 (both pairs)
-      run_op<G>(sched::OP_ELL, sub, act1, F, S1, F, T, K);
-      run_op<G>(sched::OP_ELL, sub, act, F, S2, F, T, K);
+      run_op<G, WPE>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);  // S1 | S2 (both pairs)
+      run_op<G, WPE>(sched::OP_ELL, sub, act1, F, S1, F, T, K);
+      run_op<G, WPE>(sched::OP_ELL, sub, act, F, S2, F, T, K);
     };
     for (int b = 61; b >= 0; b--) {
       step();
       if (miller_bit(b)) step();
-      run_op<G>(sched::OP_SQR, sub, act, F, 0, F, T, K);
+      run_op<G, WPE>(sched::OP_SQR, sub, act, F, 0, F, T, K);
     }
     step();
   }
-  run_op<G>(sched::OP_CONJ, sub, act, F, 0, F, T, K);
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, F, 0, F, T, K);
   if (valid && act) {
     for (int w = sub; w < 12 * NL; w += G) {
       const int e = w / NL, l = w % NL;
@@ -380,15 +510,8 @@ __device__ __forceinline__ void subgroup_rider(const i32* pts, const i32* status
   if (t < per && i < M) out[i] = g1_subgroup_status(pts, status, M, i);
 }
 
-// Lane counts whose one-workgroup-per-tile grid leaves SIMDs free at batch sizes near a
-// full GPU (8 / 9 / 12 lanes: 8, 7, 5 items per wavefront) have a second instantiation
-// of each kernel that carries the subgroup test (RIDE); its register budget is that of
-// one wavefront per SIMD, which the subgroup test's double-and-add uses.
-template <int G>
-constexpr bool rides(int which) { return which == 0 ? (G == 8 || G == 12) : (G == 9 || G == 12); }
-
 template <int G, bool RIDE>
-__global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : Cfg<G>::W) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
+__global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                                    const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                                    const int32_t* __restrict__ key_idx, int nkeys,
                                                    const i32* __restrict__ pk_soa, i32* __restrict__ status,
@@ -409,7 +532,7 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : Cfg<G>::W) void k_mille
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
     if (tile >= ntiles) break;
-    miller_tile<G>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, pk_soa, status, fout, M);
+    miller_tile<G, RIDE ? 1 : 2>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, pk_soa, status, fout, M);
   }
 }
 
@@ -423,17 +546,19 @@ constexpr size_t final_exp_lds_bytes() {
   return ((size_t)(Cfg<G>::IPW * FE_SLOTS + sched::NUM_CONSTS) * SLOT + Cfg<G>::IPW * 12) * 4;
 }
 
-template <int G>
-__device__ __forceinline__ void cyc_exp_grp(int sub, bool act, int src, int dst, int T, int K) {
-  run_op<G>(sched::OP_COPY, sub, act, src, 0, dst, T, K);
+// dst = src^|x| conjugated (= src^x on the cyclotomic subgroup).  Not inlined: one copy
+// of the straight-line cyclotomic square and product serves its five call sites.
+template <int G, int WPE>
+__device__ __noinline__ void cyc_exp_grp(int sub, bool act, int src, int dst, int T, int K) {
+  run_op<G, WPE>(sched::OP_COPY, sub, act, src, 0, dst, T, K);
   for (int i = 62; i >= 0; i--) {
-    run_op<G>(sched::OP_CYC_SQR, sub, act, dst, 0, dst, T, K);
-    if ((BLS_X >> i) & 1) run_op<G>(sched::OP_MUL, sub, act, dst, src, dst, T, K);
+    run_op_c<G, sched::OP_CYC_SQR>(sub, act, dst, 0, dst, T, K);
+    if ((BLS_X >> i) & 1) run_op_c<G, sched::OP_MUL>(sub, act, dst, src, dst, T, K);
   }
-  run_op<G>(sched::OP_CONJ, sub, act, dst, 0, dst, T, K);
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, dst, 0, dst, T, K);
 }
 
-template <int G>
+template <int G, int WPE>
 __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__ fin, const i32* __restrict__ status,
                                                const i32* __restrict__ sub_status, int32_t* __restrict__ result,
                                                u64* __restrict__ gt_out, int M) {
@@ -461,10 +586,10 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
   __syncthreads();
 
   // easy part: f^((p^6-1)(p^2+1)) with f^-1 from the norm chain Fp12 -> Fp6 -> Fp2 -> Fp
-  run_op<G>(sched::OP_N12, sub, act, R0, 0, R1, T, K);            // R1[0..5] = c0^2 - v c1^2
-  run_op<G>(sched::OP_N6A, sub, act, R1, 0, R2, T, K);            // R2[0..5] = (t0, t1, t2)
-  run_op<G>(sched::OP_N6B, sub, act, R1, R2, R3, T, K);           // R3[0..1] = sc
-  run_op<G>(sched::OP_N2, sub, act, R3, 0, R3 + 2 * SLOT, T, K);  // R3[2] = n
+  run_op<G, WPE>(sched::OP_N12, sub, act, R0, 0, R1, T, K);            // R1[0..5] = c0^2 - v c1^2
+  run_op<G, WPE>(sched::OP_N6A, sub, act, R1, 0, R2, T, K);            // R2[0..5] = (t0, t1, t2)
+  run_op<G, WPE>(sched::OP_N6B, sub, act, R1, R2, R3, T, K);           // R3[0..1] = sc
+  run_op<G, WPE>(sched::OP_N2, sub, act, R3, 0, R3 + 2 * SLOT, T, K);  // R3[2] = n
   // one inversion per workgroup (Montgomery's trick over its IPW items): thread 0 forms
   // the prefix products of the n's (inactive or zero entries count as 1), inverts the
   // total by batched divsteps (inv_fp_uniform) and peels the individual inverses off
@@ -490,38 +615,38 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
     }
   }
   __syncthreads();
-  run_op<G>(sched::OP_INV2, sub, act, R3, R3 + 3 * SLOT, R3 + 4 * SLOT, T, K);  // R3[4..5] = sc^-1
-  run_op<G>(sched::OP_INV6, sub, act, R2, R3 + 4 * SLOT, R4, T, K);             // R4[0..5] = t^-1
-  run_op<G>(sched::OP_INV12, sub, act, R0, R4, R5, T, K);                       // R5 = f^-1
-  run_op<G>(sched::OP_MUL_CONJA, sub, act, R0, R5, R1, T, K);                   // R1 = f^(p^6-1)
-  run_op<G>(sched::OP_FROB2, sub, act, R1, 0, R2, T, K);
-  run_op<G>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);                         // R2 = t2
+  run_op<G, WPE>(sched::OP_INV2, sub, act, R3, R3 + 3 * SLOT, R3 + 4 * SLOT, T, K);  // R3[4..5] = sc^-1
+  run_op<G, WPE>(sched::OP_INV6, sub, act, R2, R3 + 4 * SLOT, R4, T, K);             // R4[0..5] = t^-1
+  run_op<G, WPE>(sched::OP_INV12, sub, act, R0, R4, R5, T, K);                       // R5 = f^-1
+  run_op<G, WPE>(sched::OP_MUL_CONJA, sub, act, R0, R5, R1, T, K);                   // R1 = f^(p^6-1)
+  run_op<G, WPE>(sched::OP_FROB2, sub, act, R1, 0, R2, T, K);
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);                         // R2 = t2
 
   // hard part (api.cu:68-77).  Registers: F=R0 T1=R1 T2=R2 T3=R3 T4=R4 T0=R5
-  run_op<G>(sched::OP_COPY, sub, act, R2, 0, R0, T, K);       // f = t2
-  run_op<G>(sched::OP_CYC_SQR, sub, act, R2, 0, R1, T, K);
-  run_op<G>(sched::OP_CONJ, sub, act, R1, 0, R1, T, K);       // t1 = conj(cyc_sqr(t2))
-  cyc_exp_grp<G>(sub, act, R2, R3, T, K);                     // t3 = t2^x
-  run_op<G>(sched::OP_MUL, sub, act, R1, R3, R4, T, K);       // t4 = t1 t3
-  cyc_exp_grp<G>(sub, act, R4, R1, T, K);                     // t1 = t4^x
-  run_op<G>(sched::OP_CONJ, sub, act, R4, 0, R4, T, K);       // t4 = conj(t4)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);       // f *= t4
-  run_op<G>(sched::OP_CYC_SQR, sub, act, R3, 0, R4, T, K);    // t4 = cyc_sqr(t3)
-  cyc_exp_grp<G>(sub, act, R1, R5, T, K);                     // t0 = t1^x
-  run_op<G>(sched::OP_MUL, sub, act, R3, R5, R3, T, K);       // t3 *= t0
-  run_op<G>(sched::OP_FROB2, sub, act, R3, 0, R3, T, K);      // t3 = frob^2(t3)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);       // f *= t3
-  cyc_exp_grp<G>(sub, act, R5, R3, T, K);                     // tmp = t0^x
-  run_op<G>(sched::OP_MUL, sub, act, R4, R3, R4, T, K);       // t4 *= tmp
-  cyc_exp_grp<G>(sub, act, R4, R3, T, K);                     // tmp = t4^x
-  run_op<G>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);       // f *= tmp
-  run_op<G>(sched::OP_CONJ, sub, act, R2, 0, R5, T, K);       // conj(t2)
-  run_op<G>(sched::OP_MUL, sub, act, R4, R5, R4, T, K);       // t4 *= conj(t2)
-  run_op<G>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);       // t2 *= t1
-  run_op<G>(sched::OP_FROB3, sub, act, R2, 0, R2, T, K);      // t2 = frob^3(t2)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R2, R0, T, K);       // f *= t2
-  run_op<G>(sched::OP_FROB1, sub, act, R4, 0, R4, T, K);      // t4 = frob(t4)
-  run_op<G>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);       // f *= t4
+  run_op<G, WPE>(sched::OP_COPY, sub, act, R2, 0, R0, T, K);       // f = t2
+  run_op<G, WPE>(sched::OP_CYC_SQR, sub, act, R2, 0, R1, T, K);
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, R1, 0, R1, T, K);       // t1 = conj(cyc_sqr(t2))
+  cyc_exp_grp<G, WPE>(sub, act, R2, R3, T, K);                     // t3 = t2^x
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R1, R3, R4, T, K);       // t4 = t1 t3
+  cyc_exp_grp<G, WPE>(sub, act, R4, R1, T, K);                     // t1 = t4^x
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, R4, 0, R4, T, K);       // t4 = conj(t4)
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);       // f *= t4
+  run_op<G, WPE>(sched::OP_CYC_SQR, sub, act, R3, 0, R4, T, K);    // t4 = cyc_sqr(t3)
+  cyc_exp_grp<G, WPE>(sub, act, R1, R5, T, K);                     // t0 = t1^x
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R3, R5, R3, T, K);       // t3 *= t0
+  run_op<G, WPE>(sched::OP_FROB2, sub, act, R3, 0, R3, T, K);      // t3 = frob^2(t3)
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);       // f *= t3
+  cyc_exp_grp<G, WPE>(sub, act, R5, R3, T, K);                     // tmp = t0^x
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R4, R3, R4, T, K);       // t4 *= tmp
+  cyc_exp_grp<G, WPE>(sub, act, R4, R3, T, K);                     // tmp = t4^x
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R0, R3, R0, T, K);       // f *= tmp
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, R2, 0, R5, T, K);       // conj(t2)
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R4, R5, R4, T, K);       // t4 *= conj(t2)
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R2, R1, R2, T, K);       // t2 *= t1
+  run_op<G, WPE>(sched::OP_FROB3, sub, act, R2, 0, R2, T, K);      // t2 = frob^3(t2)
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R0, R2, R0, T, K);       // f *= t2
+  run_op<G, WPE>(sched::OP_FROB1, sub, act, R4, 0, R4, T, K);      // t4 = frob(t4)
+  run_op<G, WPE>(sched::OP_MUL, sub, act, R0, R4, R0, T, K);       // f *= t4
 
   // identity test and export: component c handled by lane c mod G
   if (in_wg) {
@@ -548,7 +673,7 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
 }
 
 template <int G, bool RIDE>
-__global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : Cfg<G>::W) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
+__global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_final_exp_grp(const i32* __restrict__ fin, const i32* __restrict__ status,
                                                       const i32* __restrict__ sub_status, int32_t* __restrict__ result,
                                                       u64* __restrict__ gt_out, int M, int* queue, SubgroupRiders riders) {
   constexpr int IPW = Cfg<G>::IPW;
@@ -565,29 +690,21 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : Cfg<G>::W) void k_final
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
     if (tile >= ntiles) break;
-    final_exp_tile<G>(tile, fin, status, sub_status, result, gt_out, M);
+    final_exp_tile<G, RIDE ? 1 : 2>(tile, fin, status, sub_status, result, gt_out, M);
   }
 }
 
 // ---------------- launch wrappers ----------------
 // Resident workgroups per CU of each kernel instantiation, queried once (callers hold
 // the library lock) so that no occupancy query happens during graph capture.
-static int num_cus() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
-  }
-  return cus;
-}
 template <int G>
-static int ml_occ() {
+int grp_occ_ml() {
   static int occ = 0;
   if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G, false>, Cfg<G>::THREADS, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
 }
 template <int G>
-static int fe_occ() {
+int grp_occ_fe() {
   static int occ = 0;
   if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G, false>, Cfg<G>::THREADS, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
@@ -596,7 +713,7 @@ static int fe_occ() {
 // fit), never more than the occupancy limit, never more than there are tiles.
 static int queue_grid(int occ, int ntiles, int wg_per_cu) {
   const int per = wg_per_cu > 0 && wg_per_cu < occ ? wg_per_cu : occ;
-  const long g = (long)num_cus() * per;
+  const long g = (long)grp_num_cus() * per;
   return (int)(g < ntiles ? g : ntiles);
 }
 
@@ -615,14 +732,14 @@ static int launch_grid(int occ, int ntiles, int threads, int* queue, int wg_per_
 }
 
 template <int G>
-static int launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                     const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                     int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
+int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
+                  const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                  int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   int grid = 0;
   if (riders.per && !rides<G>(0)) return -1;
-  if (launch_grid(queue ? ml_occ<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  if (launch_grid(queue ? grp_occ_ml<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
   if constexpr (rides<G>(0)) {
     if (riders.per) {
       hipLaunchKernelGGL((k_miller_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen,
@@ -635,13 +752,13 @@ static int launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen,
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int G>
-static int launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                     uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
+int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
+                  int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   int grid = 0;
   if (riders.per && !rides<G>(1)) return -1;
-  if (launch_grid(queue ? fe_occ<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  if (launch_grid(queue ? grp_occ_fe<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
   if constexpr (rides<G>(1)) {
     if (riders.per) {
       hipLaunchKernelGGL((k_final_exp_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), final_exp_lds_bytes<G>(), s, fin,
@@ -654,105 +771,19 @@ static int launch_fe(const int32_t* fin, const int32_t* status, const int32_t* s
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
-                      int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
-  switch (G) {
-#define ML_CASE(g) \
-  case g: return launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, riders, s);
-    ML_CASE(8) ML_CASE(12) ML_CASE(13) ML_CASE(16) ML_CASE(20) ML_CASE(24) ML_CASE(32)
-#undef ML_CASE
-    default: return -1;
-  }
-}
-int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
-  switch (G) {
-#define FE_CASE(g) case g: return launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, riders, s);
-    FE_CASE(9) FE_CASE(12) FE_CASE(16) FE_CASE(18) FE_CASE(24) FE_CASE(27) FE_CASE(32)
-#undef FE_CASE
-    default: return -1;
-  }
-}
-
-// wavefronts of a one-workgroup-per-tile launch of M items, against the GPU's SIMDs
-static int grp_waves_t(int G, int M) {
-  switch (G) {
-#define WV_CASE(g) case g: return (M + Cfg<g>::IPW - 1) / Cfg<g>::IPW * Cfg<g>::W;
-    WV_CASE(8) WV_CASE(9) WV_CASE(12) WV_CASE(13) WV_CASE(16) WV_CASE(18) WV_CASE(20) WV_CASE(24) WV_CASE(27)
-    WV_CASE(32)
-#undef WV_CASE
-    default: return 0;
-  }
-}
-bool grp_fills_simds(int G, int M) { return grp_waves_t(G, M) >= 4 * num_cus(); }
-
-static int grp_threads(int G) {
-  switch (G) {
-#define TH_CASE(g) case g: return Cfg<g>::THREADS;
-    TH_CASE(8) TH_CASE(9) TH_CASE(12) TH_CASE(13) TH_CASE(16) TH_CASE(18) TH_CASE(20) TH_CASE(24) TH_CASE(27)
-    TH_CASE(32)
-#undef TH_CASE
-    default: return 0;
-  }
-}
-// Full workgroups of riders (one item per thread), all on SIMDs that the tiles leave
-// without a wavefront.  A rider wavefront's run time (two 64-step double-and-add chains,
-// about half of a pairing wavefront's) does not lengthen the launch.
-static bool grp_rides(int which, int G) {
-  switch (G) {
-#define RD_CASE(g) case g: return rides<g>(which);
-    RD_CASE(8) RD_CASE(9) RD_CASE(12) RD_CASE(13) RD_CASE(16) RD_CASE(18) RD_CASE(20) RD_CASE(24) RD_CASE(27)
-    RD_CASE(32)
-#undef RD_CASE
-    default: return false;
-  }
-}
-int grp_rider_items(int which, int G, int M) {
-  if (M <= 0 || !grp_lanes_supported(which, G) || !grp_rides(which, G)) return 0;
-  const int threads = grp_threads(G);
-  const long free_waves = 4L * num_cus() - grp_waves_t(G, M);
-  const long rider_waves = (long)(M + threads - 1) / threads * (threads / 64);
-  return free_waves > 0 && rider_waves <= free_waves ? threads : 0;
-}
-
-bool grp_lanes_supported(int which, int G) {
-  if (which == 0) return G == 8 || G == 12 || G == 13 || G == 16 || G == 20 || G == 24 || G == 32;
-  return G == 9 || G == 12 || G == 16 || G == 18 || G == 24 || G == 27 || G == 32;
-}
-
-int occupancy_grp(int which, int G) {
-  if (which == 0) {
-    switch (G) {
-      case 8: return ml_occ<8>();
-      case 12: return ml_occ<12>();
-      case 13: return ml_occ<13>();
-      case 16: return ml_occ<16>();
-      case 20: return ml_occ<20>();
-      case 24: return ml_occ<24>();
-      case 32: return ml_occ<32>();
-      default: return -1;
-    }
-  }
-  switch (G) {
-    case 9: return fe_occ<9>();
-    case 12: return fe_occ<12>();
-    case 16: return fe_occ<16>();
-    case 18: return fe_occ<18>();
-    case 24: return fe_occ<24>();
-    case 27: return fe_occ<27>();
-    case 32: return fe_occ<32>();
-    default: return -1;
-  }
-}
-
-__global__ void k_zero_words(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-int launch_zero_words(int* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_words, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
+// the instantiations of this translation unit's lane count
+#define BLS_IS(g) || GRP_LANES == g
+#if 0 BLS_GRP_ML_LANES(BLS_IS)
+template int grp_launch_ml<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int,
+                                      const int32_t*, int32_t*, int32_t*, int, int*, int, const SubgroupRiders&,
+                                      hipStream_t);
+template int grp_occ_ml<GRP_LANES>();
+#endif
+#if 0 BLS_GRP_FE_LANES(BLS_IS)
+template int grp_launch_fe<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, int32_t*, uint64_t*, int, int*, int,
+                                      const SubgroupRiders&, hipStream_t);
+template int grp_occ_fe<GRP_LANES>();
+#endif
+#undef BLS_IS
 
 }  // namespace bls
