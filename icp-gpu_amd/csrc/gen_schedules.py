@@ -463,6 +463,20 @@ def op_ll():
         outs_of(ll.c1.c[2])
 
 
+def op_sqr_ll():
+    # f^2 (f in A) and the next step's line product LL from the scaled lines S
+    # (B 16..27) in the same rounds: f^2 at outputs 0..11, LL at 28..37 (the S slots,
+    # relative to f, written over S)
+    R = Rec()
+    sq = outs_of(f12sqr(R, f12(A)))
+    l1 = [f2(B, 8), f2(B, 9), f2(B, 10)]
+    l2 = [f2(B, 11), f2(B, 12), f2(B, 13)]
+    ll = line_product(R, l1, l2)
+    llo = outs_of(ll.c0.c[0]) + outs_of(ll.c0.c[1]) + outs_of(ll.c0.c[2]) + outs_of(ll.c1.c[1]) + \
+        outs_of(ll.c1.c[2])
+    return R, sq + [None] * 16 + llo
+
+
 def ll_operand():
     return F12(F6(f2(B, 8), f2(B, 9), f2(B, 10)), F6(ZERO2, f2(B, 11), f2(B, 12)))
 
@@ -513,6 +527,7 @@ OPS = [
     ("OP_LL", op_ll),
     ("OP_FL_SCALE", op_fl_scale),
     ("OP_FL", op_fl),
+    ("OP_SQR_LL", op_sqr_ll),
 ]
 
 

@@ -439,26 +439,32 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
     load_lines(0);
     __syncthreads();
     run_op<G, WPE>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);
-    // for b = 61 .. 0: a line step, a second one if bit b of |x| is set, the square of f;
-    // then the last line step.  One loop, so that each op's straight-line code is
-    // emitted once.
+    // L1 | L2 are free once S holds their scaled form: the raw lines of the next step
+    // are loaded before an op that ends in a barrier, ahead of the op that scales them
+    load_lines(1);
+    run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
+    // For b = 61 .. 0: a line step, a second one if bit b of |x| is set, the square of f;
+    // then the last line step.  A line step multiplies f by LL, the product of its two
+    // lines, and scales the next step's lines (OP_FL_SCALE); the next LL is formed with
+    // the square of f when one follows (OP_SQR_LL), else on its own.  One loop, so that
+    // each op's straight-line code is emitted once.
+    constexpr int STEPS = KEY_TABLE_WORDS / 84;
     bool second = false;
     for (int b = 61;;) {
-      const bool last = b < 0;
-      if (!last) load_lines(ci + 1);  // L1 | L2 are free once S holds their scaled form
-      run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
-      if (last) {
+      if (b < 0) {
         run_op<G, WPE>(sched::OP_FL, sub, act, F, L1, F, T, K);
         break;
       }
       run_op_c<G, sched::OP_FL_SCALE>(sub, act, F, L1, F, T, K);
       ci++;
+      if (ci + 1 < STEPS) load_lines(ci + 1);
       if (!second && miller_bit(b)) {
         second = true;
+        run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
         continue;
       }
       second = false;
-      run_op_c<G, sched::OP_SQR>(sub, act, F, 0, F, T, K);
+      run_op_c<G, sched::OP_SQR_LL>(sub, act, F, L1, F, T, K);
       b--;
     }
   } else {

@@ -153,8 +153,8 @@ def test_fp12_mul_and_square_against_schoolbook(variants, G):
 
 @pytest.mark.parametrize("G", [12, 16, 20, 32])
 def test_merged_line_step_against_schoolbook(variants, G):
-    """LL = line1 * line2 and f * LL (with the next step's scaled lines) equal the full
-    Fp12 products of the sparse lines x0 + x1 v + y1 v w"""
+    """LL = line1 * line2, f * LL (with the next step's scaled lines) and f^2 with the next
+    LL equal the full Fp12 products of the sparse lines x0 + x1 v + y1 v w"""
     rnd = random.Random(200 + G)
     ops = {name: (p, o) for name, p, o, _, _ in GSCH.build_rounds_v(G, variants)}
     zero = (0, 0)
@@ -182,6 +182,11 @@ def test_merged_line_step_against_schoolbook(variants, G):
             sc = [c[2], (c[1][0] * x % P, c[1][1] * x % P), (c[0][0] * y % P, c[0][1] * y % P)]
             assert out[28 + 6 * pair:34 + 6 * pair] == flat(sc)
         assert run_op(ops["OP_FL"], f, b2) == prod
+        # f^2 and the line product in one op: f^2 at 0..11, LL over the scaled lines (28..37)
+        sq = flat(PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(f)), PF.f12_from_tower(tower(f)))))
+        out = run_op(ops["OP_SQR_LL"], f, b)
+        assert out[:12] == sq and out[12:28] == [None] * 16
+        assert out[28:38] == flat([want[0], want[1], want[2], want[4], want[5]])
 
 
 def test_cyclotomic_square_on_a_cyclotomic_element(variants):
