@@ -431,24 +431,57 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
       }
     }
   };
+  // the same in two halves for the key's line (read from HBM: the generator's lines are
+  // shared by every item and stay in L2), so that it is read from memory one step before
+  // it goes to LDS: lane sub holds words sub + k G (k < LW) of the 84.  Only where the
+  // held words fit the register budget (LW <= 4, or the one-wave budget); otherwise
+  // store_lines loads the step itself.
+  constexpr int LW = (84 + G - 1) / G;
+  constexpr bool PREFETCH = LW <= 4 || (WPE == 1 && LW <= 7);
+  auto fetch_lines = [&](int ci, i32(&v)[LW]) {
+    if (PREFETCH && act) {
+#pragma unroll
+      for (int k = 0; k < LW; k++) {
+        const int w = sub + k * G;
+        if (w < 84) v[k] = pk ? pk[ci * 84 + w] : pk_soa[(size_t)(ci * 84 + w) * M + it];
+      }
+    }
+  };
+  auto store_lines = [&](int ci, const i32(&v)[LW]) {
+    if (!PREFETCH) {
+      load_lines(ci);
+    } else if (act) {
+      for (int w = sub; w < 84; w += G) g_lds[L1 + (w / NL) * SLOT + w % NL] = gen_lines[ci * 84 + w];
+#pragma unroll
+      for (int k = 0; k < LW; k++) {
+        const int w = sub + k * G;
+        if (w < 84) g_lds[L2 + (w / NL) * SLOT + w % NL] = v[k];
+      }
+    }
+  };
   int ci = 0;
   if (!__syncthreads_or(act && !act1)) {
     // Every item of the tile pairs both pairs: each step multiplies f by the product of
     // its two lines (OP_LL, then OP_FL_SCALE), and the scaling of the next step's lines
     // rides along with f * LL.  Region B of both ops is the line block L1 | L2 | P | S.
-    load_lines(0);
+    constexpr int STEPS = KEY_TABLE_WORDS / 84;
+    i32 lv[LW];
+    fetch_lines(0, lv);
+    store_lines(0, lv);
+    fetch_lines(1, lv);
     __syncthreads();
     run_op<G, WPE>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);
     // L1 | L2 are free once S holds their scaled form: the raw lines of the next step
-    // are loaded before an op that ends in a barrier, ahead of the op that scales them
-    load_lines(1);
+    // go to LDS before an op that ends in a barrier, ahead of the op that scales them,
+    // and the lines of the step after are read from memory meanwhile
+    store_lines(1, lv);
+    fetch_lines(2, lv);
     run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
     // For b = 61 .. 0: a line step, a second one if bit b of |x| is set, the square of f;
     // then the last line step.  A line step multiplies f by LL, the product of its two
     // lines, and scales the next step's lines (OP_FL_SCALE); the next LL is formed with
     // the square of f when one follows (OP_SQR_LL), else on its own.  One loop, so that
     // each op's straight-line code is emitted once.
-    constexpr int STEPS = KEY_TABLE_WORDS / 84;
     bool second = false;
     for (int b = 61;;) {
       if (b < 0) {
@@ -457,7 +490,10 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
       }
       run_op_c<G, sched::OP_FL_SCALE>(sub, act, F, L1, F, T, K);
       ci++;
-      if (ci + 1 < STEPS) load_lines(ci + 1);
+      if (ci + 1 < STEPS) {
+        store_lines(ci + 1, lv);
+        if (ci + 2 < STEPS) fetch_lines(ci + 2, lv);
+      }
       if (!second && miller_bit(b)) {
         second = true;
         run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
