@@ -21,8 +21,6 @@
 // instantiated at the end, the dispatch over G is in grp_dispatch.cpp.
 #include <hip/hip_runtime.h>
 
-#include <utility>
-
 #include "curve.hpp"
 #include "grp_config.hpp"
 #include "kernels.hpp"
@@ -93,6 +91,7 @@ __device__ __forceinline__ u32 rec_hw(const u32 (&rec)[RW], int k) {
 // loads flat loads, which the LDS wait counter also counts)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef const u32x4 __attribute__((address_space(1))) GRec;
+typedef const uint32_t __attribute__((address_space(1))) GWord;
 
 // this lane's record of a round: q 16-byte units at desc + sub * q (q is uniform)
 __device__ __forceinline__ void load_record(u32 (&rec)[REC_WORDS], GRec* __restrict__ desc, u32 at, int q,
@@ -293,22 +292,39 @@ __device__ __forceinline__ Fp load_term(const u32 (&rec)[RW], const Regions& R) 
   return lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
 }
 
+// round RI of op OP at G lanes: its shape, and this lane's record
 template <int G, int OP, int RI>
-__device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, bool active, const Regions& R,
-                                             int D) {
-  constexpr sched::OpRounds o = sched::Tab<G>::OPS_C[OP];
-  constexpr sched::Round rd = sched::Tab<G>::ROUNDS_C[o.first + RI];
-  constexpr int Q = rd.q, RW = 4 * Q;
-  u32 rec[RW];
-  GRec* p = desc + rd.desc + (u32)sub * Q;
-#pragma unroll
-  for (int u = 0; u < Q; u++) {
-    const u32x4 v = p[u];
-    rec[4 * u] = v.x;
-    rec[4 * u + 1] = v.y;
-    rec[4 * u + 2] = v.z;
-    rec[4 * u + 3] = v.w;
+struct RoundC {
+  static constexpr sched::OpRounds o = sched::Tab<G>::OPS_C[OP];
+  static constexpr sched::Round rd = sched::Tab<G>::ROUNDS_C[o.first + RI];
+  static constexpr int NR = o.np + o.no, Q = rd.q, RW = 4 * Q;
+  // whether the round reads halfword k of the record: dst, then its A and B terms
+  // (product rounds) or its terms (output rounds)
+  static constexpr bool reads(int k) {
+    if (k == 0) return true;
+    if (RI < o.np) return (k >= 1 && k < 1 + rd.na) || (k >= sched::B_AT && k < sched::B_AT + rd.nb);
+    return k < 1 + rd.na;
   }
+  // Only the words the round reads are loaded (adjacent ones merge into wide loads): a
+  // loaded word the round never reads would free its register at once, and reusing it
+  // would make the wavefront wait for the load.
+  static __device__ __forceinline__ void load(GRec* __restrict__ desc, int sub, u32 (&rec)[RW]) {
+    const GWord* p = (const GWord*)(desc + rd.desc + (u32)sub * Q);
+#pragma unroll
+    for (int w = 0; w < RW; w++) rec[w] = reads(2 * w) || reads(2 * w + 1) ? p[w] : 0u;
+  }
+};
+
+// Round RI with its record in rec; the record of round RI + 1 is read from memory before
+// this round's arithmetic, so that its latency is hidden behind it.
+template <int G, int OP, int RI>
+__device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, bool active, const Regions& R, int D,
+                                             const u32 (&rec)[RoundC<G, OP, RI>::RW]) {
+  using C = RoundC<G, OP, RI>;
+  constexpr sched::OpRounds o = C::o;
+  constexpr sched::Round rd = C::rd;
+  [[maybe_unused]] u32 nxt[RI + 1 < C::NR ? RoundC<G, OP, (RI + 1 < C::NR ? RI + 1 : RI)>::RW : 1];
+  if constexpr (RI + 1 < C::NR) RoundC<G, OP, RI + 1>::load(desc, sub, nxt);
   const u32 dst = rec[0] & 0xffffu;
   // a wavefront none of whose lanes has a product or output in this round skips it
   const bool work = __ballot(dst != sched::IDLE) != 0;
@@ -329,11 +345,7 @@ __device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, b
       if (active && dst != sched::IDLE) lds_st(D + (int)dst * SLOT, v);
     }
   }
-}
-template <int G, int OP, int... RI>
-__device__ __forceinline__ void static_rounds(std::integer_sequence<int, RI...>, GRec* __restrict__ desc, int sub,
-                                              bool active, const Regions& R, int D) {
-  (static_round<G, OP, RI>(desc, sub, active, R, D), ...);
+  if constexpr (RI + 1 < C::NR) static_round<G, OP, RI + 1>(desc, sub, active, R, D, nxt);
 }
 // run_op for an op known at compile time (same contract as run_op).  Inlined: its
 // callers hold it at one call site each inside a function entered rarely (the Miller
@@ -345,7 +357,10 @@ __device__ __forceinline__ void run_op_c(int sub, bool active, int A, int B, int
   GRec* desc = uniform_ptr((GRec*)sched::Tab<G>::desc());
   K = __builtin_amdgcn_readfirstlane(K);
   const Regions R{A, B, T, K};
-  static_rounds<G, OP>(std::make_integer_sequence<int, o.np + o.no>{}, desc, sub, active, R, D);
+  static_assert(o.np + o.no > 0, "an op has rounds");
+  u32 rec[RoundC<G, OP, 0>::RW];
+  RoundC<G, OP, 0>::load(desc, sub, rec);
+  static_round<G, OP, 0>(desc, sub, active, R, D, rec);
   __syncthreads();
 }
 
