@@ -352,16 +352,30 @@ __device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, b
 // loop's kernel, the final exponentiation's cyclotomic power), as a call of its own
 // would save and restore the callee-saved registers it uses on every op.
 template <int G, int OP>
-__device__ __forceinline__ void run_op_c(int sub, bool active, int A, int B, int D, int T, int K) {
+__device__ __forceinline__ GRec* op_desc() {
+  return uniform_ptr((GRec*)sched::Tab<G>::desc());
+}
+// the record of an op's first round, for callers that read it once ahead of a loop
+template <int G, int OP>
+struct FirstRec {
+  u32 w[RoundC<G, OP, 0>::RW];
+  __device__ __forceinline__ void load(int sub) { RoundC<G, OP, 0>::load(op_desc<G, OP>(), sub, w); }
+};
+template <int G, int OP>
+__device__ __forceinline__ void run_op_c(const FirstRec<G, OP>& first, int sub, bool active, int A, int B, int D, int T,
+                                         int K) {
   constexpr sched::OpRounds o = sched::Tab<G>::OPS_C[OP];
-  GRec* desc = uniform_ptr((GRec*)sched::Tab<G>::desc());
+  static_assert(o.np + o.no > 0, "an op has rounds");
   K = __builtin_amdgcn_readfirstlane(K);
   const Regions R{A, B, T, K};
-  static_assert(o.np + o.no > 0, "an op has rounds");
-  u32 rec[RoundC<G, OP, 0>::RW];
-  RoundC<G, OP, 0>::load(desc, sub, rec);
-  static_round<G, OP, 0>(desc, sub, active, R, D, rec);
+  static_round<G, OP, 0>(op_desc<G, OP>(), sub, active, R, D, first.w);
   __syncthreads();
+}
+template <int G, int OP>
+__device__ __forceinline__ void run_op_c(int sub, bool active, int A, int B, int D, int T, int K) {
+  FirstRec<G, OP> first;
+  first.load(sub);
+  run_op_c<G, OP>(first, sub, active, A, B, D, T, K);
 }
 
 __device__ __forceinline__ void load_consts(int K) {
@@ -498,12 +512,16 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
     // the square of f when one follows (OP_SQR_LL), else on its own.  One loop, so that
     // each op's straight-line code is emitted once.
     bool second = false;
+    FirstRec<G, sched::OP_FL_SCALE> fl0;  // first-round records, the same every step
+    FirstRec<G, sched::OP_SQR_LL> sq0;
+    fl0.load(sub);
+    sq0.load(sub);
     for (int b = 61;;) {
       if (b < 0) {
         run_op<G, WPE>(sched::OP_FL, sub, act, F, L1, F, T, K);
         break;
       }
-      run_op_c<G, sched::OP_FL_SCALE>(sub, act, F, L1, F, T, K);
+      run_op_c<G, sched::OP_FL_SCALE>(fl0, sub, act, F, L1, F, T, K);
       ci++;
       if (ci + 1 < STEPS) {
         store_lines(ci + 1, lv);
@@ -515,7 +533,7 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
         continue;
       }
       second = false;
-      run_op_c<G, sched::OP_SQR_LL>(sub, act, F, L1, F, T, K);
+      run_op_c<G, sched::OP_SQR_LL>(sq0, sub, act, F, L1, F, T, K);
       b--;
     }
   } else {
@@ -608,8 +626,10 @@ constexpr size_t final_exp_lds_bytes() {
 template <int G, int WPE>
 __device__ __noinline__ void cyc_exp_grp(int sub, bool act, int src, int dst, int T, int K) {
   run_op<G, WPE>(sched::OP_COPY, sub, act, src, 0, dst, T, K);
+  FirstRec<G, sched::OP_CYC_SQR> sq0;  // the same every iteration
+  sq0.load(sub);
   for (int i = 62; i >= 0; i--) {
-    run_op_c<G, sched::OP_CYC_SQR>(sub, act, dst, 0, dst, T, K);
+    run_op_c<G, sched::OP_CYC_SQR>(sq0, sub, act, dst, 0, dst, T, K);
     if ((BLS_X >> i) & 1) run_op_c<G, sched::OP_MUL>(sub, act, dst, src, dst, T, K);
   }
   run_op<G, WPE>(sched::OP_CONJ, sub, act, dst, 0, dst, T, K);
