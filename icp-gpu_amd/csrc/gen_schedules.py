@@ -667,12 +667,51 @@ def limbs28(v):
     return [(v >> (28 * i)) & ((1 << 28) - 1) for i in range(14)]
 
 
+# The final exponentiation's straight-line ops get a second copy of their records
+# (DESC_S, lane-major per op): a term list (operand A, operand B, or an output's terms)
+# whose busy lanes all read one region is encoded relative to that region, term = slot *
+# 14 (the slot's word offset, bits 0..9) | coefficient (signed, bits 10..15), so that the
+# kernel, knowing the region at compile time, forms the term's address with one
+# instruction.  Mixed lists keep the encoding above.  Region codes per list: 0..3 = that
+# region (A, B, T, K), 4 = mixed.  (The Miller loop's straight-line ops read DESC: its
+# kernel sits at the edge of the register budget, and a second table tips it into
+# spills.)
+STATIC_OPS = ("OP_MUL", "OP_CYC_SQR")
+MIXED = 4
+
+
+def list_region(recs, lo, n):
+    regs = set((t >> 6) & 3 for r in recs if r[0] != IDLE for t in r[lo:lo + n] if (t >> 8) & 0xFF)
+    return regs.pop() if len(regs) == 1 else MIXED
+
+
+def term_rel(t):
+    idx, coef = t & 63, (t >> 8) & 0xFF
+    if coef >= 128:
+        coef -= 256
+    assert -32 <= coef <= 31 and idx * 14 < 1024
+    return 0 if coef == 0 else (idx * 14) | ((coef & 0x3F) << 10)
+
+
+def static_record(rec, lists):
+    """rec with each (lo, n, region) list re-encoded when the region is pure"""
+    out = list(rec)
+    for lo, n, reg in lists:
+        if reg != MIXED:
+            out[lo:lo + n] = [term_rel(t) for t in rec[lo:lo + n]]
+    return out
+
+
 def emit_tables(G, ops):
-    """C++ arrays of one G: OPS<G>, ROUNDS<G>, DESC<G>"""
+    """C++ arrays of one G: OPS<G>, ROUNDS<G>, DESC<G>, and DESC_S<G> with the static
+    rounds' records (SROUNDS_C: offset into DESC_S and the region code of each list)"""
     op_rows, round_rows, words = [], [], []
+    srows, swords = [], []
     for name, prounds, orounds in ops:
         op_rows.append("{%d, %d, %d}" % (len(round_rows), len(prounds), len(orounds)))
         for k, (na, nb, recs) in enumerate(prounds + orounds):
+            if name not in STATIC_OPS:
+                srows.append("{0xffffffffu, 0, %d, %d}" % (MIXED, MIXED))
             q = len(recs[0]) // 8
             flags = 0
             if k < len(prounds):
@@ -687,16 +726,45 @@ def emit_tables(G, ops):
             for rec in recs:
                 for k in range(0, len(rec), 2):
                     words.append(rec[k] | (rec[k + 1] << 16))
+        if name in STATIC_OPS:
+            # lane-major: lane s's records of all the op's rounds are contiguous, so that a
+            # lane addresses every round from one base with a small immediate offset
+            rounds = prounds + orounds
+            stride = sum(len(recs[0]) // 8 for _, _, recs in rounds)
+            base = len(swords) // 4
+            off = 0
+            lane_words = [[] for _ in range(G)]
+            for k, (na, nb, recs) in enumerate(rounds):
+                def reg(lo, n):
+                    return list_region(recs, lo, n)
+                if k < len(prounds):
+                    lists = [(1, na, reg(1, na)), (B_AT, nb, reg(B_AT, nb))]
+                else:
+                    lists = [(1, na, reg(1, na))]
+                srows.append("{%d, %d, %d, %d}" % (base + off, stride, lists[0][2],
+                                                   lists[1][2] if len(lists) > 1 else MIXED))
+                for sl, rec in enumerate(recs):
+                    srec = static_record(rec, lists)
+                    lane_words[sl] += [srec[j] | (srec[j + 1] << 16) for j in range(0, len(srec), 2)]
+                off += len(recs[0]) // 8
+            for sl in range(G):
+                assert len(lane_words[sl]) == 4 * stride
+                swords += lane_words[sl]
     assert len(round_rows) < 65536
     out = ["// lanes per item G = %d: %d rounds, %d bytes of records" % (G, len(round_rows), 4 * len(words)),
            "__constant__ const OpRounds OPS%d[NUM_OPS] = {%s};" % (G, ", ".join(op_rows)),
            "__constant__ const Round ROUNDS%d[%d] = {%s};" % (G, len(round_rows), ", ".join(round_rows)),
            "__device__ __align__(16) const uint32_t DESC%d[%d] = {%s};" % (
                G, len(words), ", ".join("0x%x" % w for w in words)),
+           "__device__ __align__(16) const uint32_t DESC_S%d[%d] = {%s};" % (
+               G, len(swords), ", ".join("0x%x" % w for w in swords)),
            "template <> struct Tab<%d> {" % G,
            "  // the same tables as compile-time constants (round shapes of the ops run as straight-line code)",
            "  static constexpr OpRounds OPS_C[NUM_OPS] = {%s};" % ", ".join(op_rows),
            "  static constexpr Round ROUNDS_C[%d] = {%s};" % (len(round_rows), ", ".join(round_rows)),
+           "  static constexpr SRound SROUNDS_C[%d] = {%s};" % (len(srows), ", ".join(srows)),
+           "  static __device__ __forceinline__ const uint4* sdesc() { return reinterpret_cast<const uint4*>(DESC_S%d); }"
+           % G,
            "  static __device__ __forceinline__ const COpRounds* ops() { return (const COpRounds*)OPS%d; }" % G,
            "  static __device__ __forceinline__ const CRound* rounds() { return (const CRound*)ROUNDS%d; }" % G,
            "  static __device__ __forceinline__ const uint4* desc() { return reinterpret_cast<const uint4*>(DESC%d); }"
@@ -727,6 +795,12 @@ def main():
     lines.append("// flags (product rounds): 1 = operand A is one slot with coefficient 1 on every")
     lines.append("// busy lane, 2 = the same for B")
     lines.append("struct Round { uint32_t desc; uint8_t na, nb, q, flags; };")
+    lines.append("// the straight-line rounds' records in DESC_S, lane-major per op: lane s's record of")
+    lines.append("// the round is at desc + s * stride (16-byte units); the region code of operand A (or")
+    lines.append("// an output round's terms) and of operand B: 0..3 = all terms in region A, B, T, K,")
+    lines.append("// encoded slot * 14 | coefficient << 10; 4 = mixed regions, terms as in DESC")
+    lines.append("struct SRound { uint32_t desc; uint16_t stride; uint8_t ra, rb; };")
+    lines.append("constexpr int MIXED = %d;" % MIXED)
     lines.append("// the op and round tables sit in the constant address space, so that loads at")
     lines.append("// uniform indices are scalar loads")
     lines.append("typedef const OpRounds __attribute__((address_space(4))) COpRounds;")

@@ -17,8 +17,9 @@
 // consecutive words per item), so the G lanes of an item read and write whole cache
 // lines whichever XCD runs the workgroup.
 //
-// This file is compiled once per lane count (-DGRP_LANES=G); the launchers of each G are
-// instantiated at the end, the dispatch over G is in grp_dispatch.cpp.
+// This file is compiled once per lane count and kernel (-DGRP_LANES=G -DGRP_KIND=0 for
+// the Miller loop, 1 for the final exponentiation); the launchers are instantiated at
+// the end, the dispatch over G is in grp_dispatch.cpp.
 #include <hip/hip_runtime.h>
 
 #include "curve.hpp"
@@ -26,8 +27,8 @@
 #include "kernels.hpp"
 #include "schedules_gen.hpp"
 
-#ifndef GRP_LANES
-#error "kernels_grp.hip is compiled once per lane count: -DGRP_LANES=<G>"
+#if !defined(GRP_LANES) || !defined(GRP_KIND)
+#error "kernels_grp.hip is compiled once per lane count and kernel: -DGRP_LANES=<G> -DGRP_KIND=<0|1>"
 #endif
 
 namespace bls {
@@ -70,6 +71,7 @@ __device__ __forceinline__ void st_ref_words(u64* p, const Fp& a) {
 struct Regions {
   int a, b, t, k;
   __device__ __forceinline__ int base(int reg) const { return reg == 0 ? a : reg == 1 ? b : reg == 2 ? t : k; }
+  static_assert(SLOT == 14, "region-relative terms hold slot * 14");
   // the same, branch-free (reg differs between lanes)
   __device__ __forceinline__ int select(u32 reg) const {
     const int m1 = -(int)(reg & 1), m2 = -(int)((reg >> 1) & 1);
@@ -254,8 +256,24 @@ __device__ __forceinline__ int opaque_one() {
   return v;
 }
 
+
+// Term k of the record: its slot's LDS word offset and its coefficient.  REG is the
+// region of every term of the list (the slot offset relative to it is in the record)
+// or sched::MIXED (region, slot and coefficient fields as in the interpreter's tables).
+template <int REG, int RW>
+__device__ __forceinline__ int term_at(const u32 (&rec)[RW], int k, const Regions& R, i32& coef) {
+  const u32 t = rec_hw(rec, k);
+  if constexpr (REG == sched::MIXED) {
+    coef = (i32)(int8_t)(t >> 8);
+    return R.select(t >> 6) + (int)(t & 63) * SLOT;
+  } else {
+    coef = ((i32)(t << 16)) >> 26;  // bits 10..15, signed
+    return R.base(REG) + (int)(t & 1023u);
+  }
+}
+
 // the sum of the N >= 1 terms at halfwords FIRST.. of the record, carried
-template <int FIRST, int N, int RW>
+template <int FIRST, int N, int REG, int RW>
 __device__ __forceinline__ Fp sum_terms(const u32 (&rec)[RW], const Regions& R) {
   static_assert(N >= 1 && FIRST + N <= 2 * RW, "terms of a busy round");
   i64 acc[NL];
@@ -266,11 +284,7 @@ __device__ __forceinline__ Fp sum_terms(const u32 (&rec)[RW], const Regions& R) 
       i32 k[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        if (c + j < N) {
-          const u32 t = rec_hw(rec, FIRST + c + j);
-          x[j] = lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
-          k[j] = (i32)(int8_t)(t >> 8);
-        }
+        if (c + j < N) x[j] = lds_ld(term_at<REG>(rec, FIRST + c + j, R, k[j]));
       }
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -286,10 +300,11 @@ __device__ __forceinline__ Fp sum_terms(const u32 (&rec)[RW], const Regions& R) 
   }
   return carry64(acc);
 }
-template <int AT, int RW>
+// a one-term operand with coefficient 1 on every busy lane
+template <int AT, int REG, int RW>
 __device__ __forceinline__ Fp load_term(const u32 (&rec)[RW], const Regions& R) {
-  const u32 t = rec_hw(rec, AT);
-  return lds_ld(R.select(t >> 6) + (int)(t & 63) * SLOT);
+  i32 coef;
+  return lds_ld(term_at<REG>(rec, AT, R, coef));
 }
 
 // round RI of op OP at G lanes: its shape, and this lane's record
@@ -297,6 +312,10 @@ template <int G, int OP, int RI>
 struct RoundC {
   static constexpr sched::OpRounds o = sched::Tab<G>::OPS_C[OP];
   static constexpr sched::Round rd = sched::Tab<G>::ROUNDS_C[o.first + RI];
+  // records in DESC_S (region-relative terms, lane-major) or in the interpreter's DESC
+  static constexpr sched::SRound s0 = sched::Tab<G>::SROUNDS_C[o.first + RI];
+  static constexpr bool S = s0.desc != 0xffffffffu;
+  static constexpr sched::SRound sr = S ? s0 : sched::SRound{rd.desc, rd.q, sched::MIXED, sched::MIXED};
   static constexpr int NR = o.np + o.no, Q = rd.q, RW = 4 * Q;
   // whether the round reads halfword k of the record: dst, then its A and B terms
   // (product rounds) or its terms (output rounds)
@@ -309,7 +328,7 @@ struct RoundC {
   // loaded word the round never reads would free its register at once, and reusing it
   // would make the wavefront wait for the load.
   static __device__ __forceinline__ void load(GRec* __restrict__ desc, int sub, u32 (&rec)[RW]) {
-    const GWord* p = (const GWord*)(desc + rd.desc + (u32)sub * Q);
+    const GWord* p = (const GWord*)(desc + sr.desc + (u32)sub * sr.stride);
 #pragma unroll
     for (int w = 0; w < RW; w++) rec[w] = reads(2 * w) || reads(2 * w + 1) ? p[w] : 0u;
   }
@@ -331,17 +350,17 @@ __device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, b
   if constexpr (RI < o.np) {
     if (work) {
       Fp a, b;
-      if constexpr (rd.flags & 1) a = load_term<1>(rec, R);
-      else a = sum_terms<1, rd.na>(rec, R);
-      if constexpr (rd.flags & 2) b = load_term<sched::B_AT>(rec, R);
-      else b = sum_terms<sched::B_AT, rd.nb>(rec, R);
+      if constexpr (rd.flags & 1) a = load_term<1, C::sr.ra>(rec, R);
+      else a = sum_terms<1, rd.na, C::sr.ra>(rec, R);
+      if constexpr (rd.flags & 2) b = load_term<sched::B_AT, C::sr.rb>(rec, R);
+      else b = sum_terms<sched::B_AT, rd.nb, C::sr.rb>(rec, R);
       const Fp pr = mul(a, b);
       if (active && dst != sched::IDLE) lds_st(R.t + (int)dst * SLOT, pr);
     }
     if constexpr (RI + 1 == o.np) __syncthreads();
   } else {
     if (work) {
-      const Fp v = fold_top(sum_terms<1, rd.na>(rec, R));
+      const Fp v = fold_top(sum_terms<1, rd.na, C::sr.ra>(rec, R));
       if (active && dst != sched::IDLE) lds_st(D + (int)dst * SLOT, v);
     }
   }
@@ -353,7 +372,8 @@ __device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, b
 // would save and restore the callee-saved registers it uses on every op.
 template <int G, int OP>
 __device__ __forceinline__ GRec* op_desc() {
-  return uniform_ptr((GRec*)sched::Tab<G>::desc());
+  if constexpr (RoundC<G, OP, 0>::S) return uniform_ptr((GRec*)sched::Tab<G>::sdesc());
+  else return uniform_ptr((GRec*)sched::Tab<G>::desc());
 }
 // the record of an op's first round, for callers that read it once ahead of a loop
 template <int G, int OP>
@@ -848,15 +868,17 @@ int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// the instantiations of this translation unit's lane count
+// the instantiations of this translation unit: lane count GRP_LANES, kernel GRP_KIND
+// (0 Miller, 1 final exponentiation; one kernel per unit, so that neither kernel's
+// register allocation depends on the other's code)
 #define BLS_IS(g) || GRP_LANES == g
-#if 0 BLS_GRP_ML_LANES(BLS_IS)
+#if GRP_KIND == 0 && (0 BLS_GRP_ML_LANES(BLS_IS))
 template int grp_launch_ml<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int,
                                       const int32_t*, int32_t*, int32_t*, int, int*, int, const SubgroupRiders&,
                                       hipStream_t);
 template int grp_occ_ml<GRP_LANES>();
 #endif
-#if 0 BLS_GRP_FE_LANES(BLS_IS)
+#if GRP_KIND == 1 && (0 BLS_GRP_FE_LANES(BLS_IS))
 template int grp_launch_fe<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, int32_t*, uint64_t*, int, int*, int,
                                       const SubgroupRiders&, hipStream_t);
 template int grp_occ_fe<GRP_LANES>();

@@ -208,6 +208,68 @@ def test_cyclotomic_square_on_a_cyclotomic_element(variants):
     assert run_op(ops["OP_CYC_SQR"], flat(f), [0] * 28) != flat(PF.f12_to_tower(PF.f12mul_w(fw, fw)))
 
 
+def _array(text, decl):
+    body = text.split(decl, 1)[1].split("= {", 1)[1].split("};", 1)[0]
+    return body
+
+
+def _rows(body):
+    return [[int(x, 0) for x in r.strip("{} ").replace("u", "").split(",")] for r in body.split("}, {")]
+
+
+@pytest.mark.parametrize("G", GSCH.GS)
+def test_static_tables_execute_the_final_exponentiation_ops(variants, G):
+    """DESC_S as the straight-line rounds read it: lane-major records per op, and term lists
+    whose busy lanes read one region encoded relative to it (slot * 14, 6-bit signed
+    coefficient).  Executed here, every such op gives the values of its symbolic
+    definition."""
+    path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
+    text = open(path).read()
+    tab = text.split("template <> struct Tab<%d> {" % G, 1)[1].split("\n};", 1)[0]
+    ops_c = _rows(_array(tab, "OPS_C["))
+    rounds_c = _rows(_array(tab, "ROUNDS_C["))
+    srounds = _rows(_array(tab, "SROUNDS_C["))
+    words = [int(x, 0) for x in _array(text, "DESC_S%d[" % G).split(",")]
+    names = [n for n, _, _, _ in variants[0]]
+    progs = {n: (prods, outs) for n, _, _, prods, outs in GSCH.build_rounds_v(G, variants)}
+    rnd = random.Random(300 + G)
+    for name in GSCH.STATIC_OPS:
+        first, np_, no = ops_c[names.index(name)]
+        prods, outs = progs[name]
+        for _ in range(2):
+            a = [rnd.randrange(P) for _ in range(12)]
+            b = [rnd.randrange(P) for _ in range(28)]
+            regions = {A: a, B: b, T: {}}
+
+            def term(t, reg):
+                if reg == GSCH.MIXED:
+                    r, idx, coef = decode(t)
+                else:
+                    off, coef = t & 1023, (t >> 10) & 63
+                    coef -= 64 if coef >= 32 else 0
+                    assert off % 14 == 0
+                    r, idx = reg, off // 14
+                return 0 if coef == 0 else coef * regions[r][idx]
+
+            out = {}
+            for k in range(np_ + no):
+                desc, na, nb, q, flags = rounds_c[first + k]
+                sdesc, stride, ra, rb = srounds[first + k]
+                for lane in range(G):
+                    at = 4 * (sdesc + lane * stride)
+                    hw = [h for w in words[at:at + 4 * q] for h in (w & 0xFFFF, w >> 16)]
+                    if hw[0] == GSCH.IDLE:
+                        continue
+                    if k < np_:
+                        x = sum(term(t, ra) for t in hw[1:1 + na])
+                        y = sum(term(t, rb) for t in hw[GSCH.B_AT:GSCH.B_AT + nb])
+                        regions[T][hw[0]] = x * y % P
+                    else:
+                        out[hw[0]] = sum(term(t, ra) for t in hw[1:1 + na]) % P
+            got = [out.get(i) for i in range(max(out) + 1)]
+            assert got == symbolic(prods, outs, a, b), (G, name)
+
+
 def test_generated_header_is_current(variants):
     """schedules_gen.hpp is what the generator writes now (the build does not regenerate it)"""
     path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
