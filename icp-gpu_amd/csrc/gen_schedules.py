@@ -673,10 +673,16 @@ def limbs28(v):
 # 14 (the slot's word offset, bits 0..9) | coefficient (signed, bits 10..15), so that the
 # kernel, knowing the region at compile time, forms the term's address with one
 # instruction.  Mixed lists keep the encoding above.  Region codes per list: 0..3 = that
-# region (A, B, T, K), 4 = mixed.  (The Miller loop's straight-line ops read DESC: its
-# kernel sits at the edge of the register budget, and a second table tips it into
-# spills.)
-STATIC_OPS = ("OP_MUL", "OP_CYC_SQR")
+# region (A, B, T, K), 4 = mixed.  The Miller loop's straight-line ops use DESC_S only at
+# the lane counts where that measured faster: at 8, 13 and 16 lanes the kernel, at the
+# edge of its register budget, spills heavily with it and runs slower.
+FE_STATIC_OPS = ("OP_MUL", "OP_CYC_SQR")
+ML_STATIC_OPS = ("OP_LL", "OP_FL_SCALE", "OP_SQR_LL")
+ML_STATIC_LANES = (12, 20, 24, 32)
+
+
+def static_ops(G):
+    return FE_STATIC_OPS + (ML_STATIC_OPS if G in ML_STATIC_LANES else ())
 MIXED = 4
 
 
@@ -710,7 +716,7 @@ def emit_tables(G, ops):
     for name, prounds, orounds in ops:
         op_rows.append("{%d, %d, %d}" % (len(round_rows), len(prounds), len(orounds)))
         for k, (na, nb, recs) in enumerate(prounds + orounds):
-            if name not in STATIC_OPS:
+            if name not in static_ops(G):
                 srows.append("{0xffffffffu, 0, %d, %d}" % (MIXED, MIXED))
             q = len(recs[0]) // 8
             flags = 0
@@ -726,7 +732,7 @@ def emit_tables(G, ops):
             for rec in recs:
                 for k in range(0, len(rec), 2):
                     words.append(rec[k] | (rec[k + 1] << 16))
-        if name in STATIC_OPS:
+        if name in static_ops(G):
             # lane-major: lane s's records of all the op's rounds are contiguous, so that a
             # lane addresses every round from one base with a small immediate offset
             rounds = prounds + orounds

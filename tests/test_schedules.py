@@ -218,7 +218,7 @@ def _rows(body):
 
 
 @pytest.mark.parametrize("G", GSCH.GS)
-def test_static_tables_execute_the_final_exponentiation_ops(variants, G):
+def test_static_tables_execute_the_straight_line_ops(variants, G):
     """DESC_S as the straight-line rounds read it: lane-major records per op, and term lists
     whose busy lanes read one region encoded relative to it (slot * 14, 6-bit signed
     coefficient).  Executed here, every such op gives the values of its symbolic
@@ -233,7 +233,7 @@ def test_static_tables_execute_the_final_exponentiation_ops(variants, G):
     names = [n for n, _, _, _ in variants[0]]
     progs = {n: (prods, outs) for n, _, _, prods, outs in GSCH.build_rounds_v(G, variants)}
     rnd = random.Random(300 + G)
-    for name in GSCH.STATIC_OPS:
+    for name in GSCH.static_ops(G):
         first, np_, no = ops_c[names.index(name)]
         prods, outs = progs[name]
         for _ in range(2):
