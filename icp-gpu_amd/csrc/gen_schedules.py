@@ -686,25 +686,41 @@ def static_ops(G):
 MIXED = 4
 
 
+# A list whose busy lanes read two regions r0 < r1 has code PAIR + 4 r0 + r1 and terms
+# slot * 14 (bits 0..9) | (region == r1) << 10 | coefficient (signed, bits 11..15).
+PAIR = 8
+
+
 def list_region(recs, lo, n):
-    regs = set((t >> 6) & 3 for r in recs if r[0] != IDLE for t in r[lo:lo + n] if (t >> 8) & 0xFF)
-    return regs.pop() if len(regs) == 1 else MIXED
+    regs = sorted(set((t >> 6) & 3 for r in recs if r[0] != IDLE for t in r[lo:lo + n] if (t >> 8) & 0xFF))
+    if len(regs) == 1:
+        return regs[0]
+    if len(regs) == 2:
+        return PAIR + 4 * regs[0] + regs[1]
+    return MIXED
 
 
-def term_rel(t):
-    idx, coef = t & 63, (t >> 8) & 0xFF
+def term_rel(t, code):
+    idx, reg, coef = t & 63, (t >> 6) & 3, (t >> 8) & 0xFF
     if coef >= 128:
         coef -= 256
-    assert -32 <= coef <= 31 and idx * 14 < 1024
-    return 0 if coef == 0 else (idx * 14) | ((coef & 0x3F) << 10)
+    if coef == 0:
+        return 0
+    assert idx * 14 < 1024
+    if code < MIXED:
+        assert reg == code and -32 <= coef <= 31
+        return (idx * 14) | ((coef & 0x3F) << 10)
+    r0, r1 = (code - PAIR) // 4, (code - PAIR) % 4
+    assert reg in (r0, r1) and -16 <= coef <= 15
+    return (idx * 14) | ((reg == r1) << 10) | ((coef & 0x1F) << 11)
 
 
 def static_record(rec, lists):
-    """rec with each (lo, n, region) list re-encoded when the region is pure"""
+    """rec with each (lo, n, region code) list re-encoded unless its code is MIXED"""
     out = list(rec)
-    for lo, n, reg in lists:
-        if reg != MIXED:
-            out[lo:lo + n] = [term_rel(t) for t in rec[lo:lo + n]]
+    for lo, n, code in lists:
+        if code != MIXED:
+            out[lo:lo + n] = [term_rel(t, code) for t in rec[lo:lo + n]]
     return out
 
 
@@ -804,9 +820,10 @@ def main():
     lines.append("// the straight-line rounds' records in DESC_S, lane-major per op: lane s's record of")
     lines.append("// the round is at desc + s * stride (16-byte units); the region code of operand A (or")
     lines.append("// an output round's terms) and of operand B: 0..3 = all terms in region A, B, T, K,")
-    lines.append("// encoded slot * 14 | coefficient << 10; 4 = mixed regions, terms as in DESC")
+    lines.append("// encoded slot * 14 | coefficient << 10; 8 + 4 r0 + r1 = terms in regions r0 < r1,")
+    lines.append("// slot * 14 | (region == r1) << 10 | coefficient << 11; 4 = terms as in DESC")
     lines.append("struct SRound { uint32_t desc; uint16_t stride; uint8_t ra, rb; };")
-    lines.append("constexpr int MIXED = %d;" % MIXED)
+    lines.append("constexpr int MIXED = %d, PAIR = %d;" % (MIXED, PAIR))
     lines.append("// the op and round tables sit in the constant address space, so that loads at")
     lines.append("// uniform indices are scalar loads")
     lines.append("typedef const OpRounds __attribute__((address_space(4))) COpRounds;")

@@ -266,6 +266,10 @@ __device__ __forceinline__ int term_at(const u32 (&rec)[RW], int k, const Region
   if constexpr (REG == sched::MIXED) {
     coef = (i32)(int8_t)(t >> 8);
     return R.select(t >> 6) + (int)(t & 63) * SLOT;
+  } else if constexpr (REG >= sched::PAIR) {  // two regions: bit 10 picks the second
+    coef = ((i32)(t << 16)) >> 27;  // bits 11..15, signed
+    const int b0 = R.base((REG - sched::PAIR) / 4), b1 = R.base((REG - sched::PAIR) % 4);
+    return ((t & 1024u) ? b1 : b0) + (int)(t & 1023u);
   } else {
     coef = ((i32)(t << 16)) >> 26;  // bits 10..15, signed
     return R.base(REG) + (int)(t & 1023u);

@@ -219,10 +219,10 @@ def _rows(body):
 
 @pytest.mark.parametrize("G", GSCH.GS)
 def test_static_tables_execute_the_straight_line_ops(variants, G):
-    """DESC_S as the straight-line rounds read it: lane-major records per op, and term lists
+    """DESC_S as the straight-line rounds read it: lane-major records per op, term lists
     whose busy lanes read one region encoded relative to it (slot * 14, 6-bit signed
-    coefficient).  Executed here, every such op gives the values of its symbolic
-    definition."""
+    coefficient) and lists over two regions with a region bit (5-bit coefficient).
+    Executed here, every such op gives the values of its symbolic definition."""
     path = os.path.join(os.path.dirname(HERE), "icp-gpu_amd", "csrc", "schedules_gen.hpp")
     text = open(path).read()
     tab = text.split("template <> struct Tab<%d> {" % G, 1)[1].split("\n};", 1)[0]
@@ -244,6 +244,12 @@ def test_static_tables_execute_the_straight_line_ops(variants, G):
             def term(t, reg):
                 if reg == GSCH.MIXED:
                     r, idx, coef = decode(t)
+                elif reg >= GSCH.PAIR:
+                    off, coef = t & 1023, (t >> 11) & 31
+                    coef -= 32 if coef >= 16 else 0
+                    r = (reg - GSCH.PAIR) % 4 if t & 1024 else (reg - GSCH.PAIR) // 4
+                    assert off % 14 == 0
+                    idx = off // 14
                 else:
                     off, coef = t & 1023, (t >> 10) & 63
                     coef -= 64 if coef >= 32 else 0
