@@ -21,18 +21,36 @@ struct Aff {
   F x, y;
 };
 
+// The field products of the group law: inline (FieldInline), or through one out-of-line
+// multiplier and squarer (FpCalls, Fp only), which keeps the code of a long
+// double-and-add loop small (the subgroup test runs beside the pairing kernels, on the
+// CUs whose instruction cache their loops use).
+struct FieldInline {
+  template <class F>
+  static __device__ __forceinline__ F mul(const F& a, const F& b) { return bls::mul(a, b); }
+  template <class F>
+  static __device__ __forceinline__ F sqr(const F& a) { return sqr_any(a); }
+};
+// (operands by value: they travel in registers, not through the stack)
+static __device__ __noinline__ Fp mul_call(Fp a, Fp b) { return mul(a, b); }
+static __device__ __noinline__ Fp sqr_call(Fp a) { return sqr(a); }
+struct FpCalls {
+  static __device__ __forceinline__ Fp mul(const Fp& a, const Fp& b) { return mul_call(a, b); }
+  static __device__ __forceinline__ Fp sqr(const Fp& a) { return sqr_call(a); }
+};
+
 // dbl-2009-l (a = 0).  Z = 0 (identity) maps to Z3 = 0.
-template <class F>
+template <class F, class M = FieldInline>
 __device__ __forceinline__ Jac<F> jdbl(const Jac<F>& p) {
-  F A = sqr_any(p.x), B = sqr_any(p.y), C = sqr_any(B);
-  F D = norm(dbl(sub(sub(sqr_any(carry(add(p.x, B))), A), C)));
+  F A = M::sqr(p.x), B = M::sqr(p.y), C = M::sqr(B);
+  F D = norm(dbl(sub(sub(M::sqr(carry(add(p.x, B))), A), C)));
   F E = norm(add(dbl(A), A));
-  F Fv = sqr_any(E);
+  F Fv = M::sqr(E);
   Jac<F> r;
   r.x = norm(sub(Fv, dbl(D)));
   F C8 = norm(dbl(dbl(dbl(C))));  // 8C: normalized before the subtraction (limb range)
-  r.y = norm(sub(mul(E, sub(D, r.x)), C8));
-  r.z = norm(dbl(mul(p.y, p.z)));
+  r.y = norm(sub(M::mul(E, sub(D, r.x)), C8));
+  r.z = norm(dbl(M::mul(p.y, p.z)));
   return r;
 }
 // madd-2007-bl: p Jacobian, q affine (not the identity).
@@ -117,33 +135,35 @@ enum DecodeStatus : int32_t {
 };
 
 // Jacobian + Jacobian (add-2007-bl), a = 0, with the doubling and identity cases.
+template <class M = FieldInline>
 __device__ __forceinline__ Jac<Fp> jadd_jac(const Jac<Fp>& p, const Jac<Fp>& q) {
   if (is_zero(p.z)) return q;
   if (is_zero(q.z)) return p;
-  Fp z1z1 = sqr(p.z), z2z2 = sqr(q.z);
-  Fp u1 = mul(p.x, z2z2), u2 = mul(q.x, z1z1);
-  Fp s1 = mul(p.y, mul(q.z, z2z2)), s2 = mul(q.y, mul(p.z, z1z1));
+  Fp z1z1 = M::sqr(p.z), z2z2 = M::sqr(q.z);
+  Fp u1 = M::mul(p.x, z2z2), u2 = M::mul(q.x, z1z1);
+  Fp s1 = M::mul(p.y, M::mul(q.z, z2z2)), s2 = M::mul(q.y, M::mul(p.z, z1z1));
   Fp h = norm(sub(u2, u1));
   Fp r = norm(dbl(sub(s2, s1)));
   if (is_zero(h)) {
-    if (is_zero(r)) return jdbl(p);
+    if (is_zero(r)) return jdbl<Fp, M>(p);
     return {fp_one(), fp_one(), fp_zero()};
   }
-  Fp i = sqr(norm(dbl(h)));
-  Fp j = mul(h, i);
-  Fp v = mul(u1, i);
+  Fp i = M::sqr(norm(dbl(h)));
+  Fp j = M::mul(h, i);
+  Fp v = M::mul(u1, i);
   Jac<Fp> o;
-  o.x = norm(sub(sub(sub(sqr(r), j), v), v));
-  o.y = norm(sub(mul(r, sub(v, o.x)), dbl(mul(s1, j))));
-  o.z = mul(norm(sub(sub(sqr(carry(add(p.z, q.z))), z1z1), z2z2)), h);
+  o.x = norm(sub(sub(sub(M::sqr(r), j), v), v));
+  o.y = norm(sub(M::mul(r, sub(v, o.x)), dbl(M::mul(s1, j))));
+  o.z = M::mul(norm(sub(sub(M::sqr(carry(add(p.z, q.z))), z1z1), z2z2)), h);
   return o;
 }
+template <class M = FieldInline>
 __device__ __forceinline__ Jac<Fp> mul_absx_jac(const Jac<Fp>& q) {
   Jac<Fp> acc = q;
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
-    acc = jdbl(acc);
-    if ((BLS_X >> i) & 1) acc = jadd_jac(acc, q);
+    acc = jdbl<Fp, M>(acc);
+    if ((BLS_X >> i) & 1) acc = jadd_jac<M>(acc, q);
   }
   return acc;
 }
@@ -159,16 +179,19 @@ __device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
 // G1, DEC_NOT_IN_SUBGROUP otherwise.  [x^2]P is computed as [|x|]([|x|]P) without
 // leaving Jacobian coordinates, and compared with (beta x, -y) projectively:
 // X == beta x Z^2 and Y == -y Z^3 (the test of g1_in_subgroup).
+// The field products go through FpCalls: the test is latency work on few wavefronts,
+// and one copy of the multiplier keeps its loop small.
 __device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32* status, int M, int i) {
   if (status[i] != DEC_OK) return DEC_OK;
   const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
   Jac<Fp> p1 = {x, y, fp_one()};
-  Jac<Fp> a = mul_absx_jac(p1);
-  Jac<Fp> b = mul_absx_jac(a);
+  Jac<Fp> a = mul_absx_jac<FpCalls>(p1);
+  Jac<Fp> b = mul_absx_jac<FpCalls>(a);
   bool in = false;
   if (!is_zero(b.z)) {
-    Fp z2 = sqr(b.z);
-    in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
+    Fp z2 = FpCalls::sqr(b.z);
+    in = eq(b.x, FpCalls::mul(FpCalls::mul(x, fp_const(BETA_M)), z2)) &&
+         eq(b.y, neg(FpCalls::mul(y, FpCalls::mul(z2, b.z))));
   }
   return in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }

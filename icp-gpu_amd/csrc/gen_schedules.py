@@ -733,7 +733,7 @@ def emit_tables(G, ops):
         op_rows.append("{%d, %d, %d}" % (len(round_rows), len(prounds), len(orounds)))
         for k, (na, nb, recs) in enumerate(prounds + orounds):
             if name not in static_ops(G):
-                srows.append("{0xffffffffu, 0, %d, %d}" % (MIXED, MIXED))
+                srows.append("{0xffffffffu, 0, %d, %d, 0, 0}" % (MIXED, MIXED))
             q = len(recs[0]) // 8
             flags = 0
             if k < len(prounds):
@@ -763,8 +763,20 @@ def emit_tables(G, ops):
                     lists = [(1, na, reg(1, na)), (B_AT, nb, reg(B_AT, nb))]
                 else:
                     lists = [(1, na, reg(1, na))]
-                srows.append("{%d, %d, %d, %d}" % (base + off, stride, lists[0][2],
-                                                   lists[1][2] if len(lists) > 1 else MIXED))
+                # largest sum of |coefficients| of each operand list over the busy lanes: a
+                # sum of weight <= 2 of normalized slots is a valid multiplier operand as
+                # it is (no carry pass)
+                def weight(lo, n):
+                    w = 0
+                    for r in recs:
+                        if r[0] != IDLE:
+                            w = max(w, sum(abs(((t >> 8) & 0xFF) - (256 if (t >> 8) & 0x80 else 0))
+                                           for t in r[lo:lo + n]))
+                    return w
+                wa = weight(1, na) if k < len(prounds) else 0
+                wb = weight(B_AT, nb) if k < len(prounds) else 0
+                srows.append("{%d, %d, %d, %d, %d, %d}" % (base + off, stride, lists[0][2],
+                                                           lists[1][2] if len(lists) > 1 else MIXED, wa, wb))
                 for sl, rec in enumerate(recs):
                     srec = static_record(rec, lists)
                     lane_words[sl] += [srec[j] | (srec[j + 1] << 16) for j in range(0, len(srec), 2)]
@@ -822,7 +834,8 @@ def main():
     lines.append("// an output round's terms) and of operand B: 0..3 = all terms in region A, B, T, K,")
     lines.append("// encoded slot * 14 | coefficient << 10; 8 + 4 r0 + r1 = terms in regions r0 < r1,")
     lines.append("// slot * 14 | (region == r1) << 10 | coefficient << 11; 4 = terms as in DESC")
-    lines.append("struct SRound { uint32_t desc; uint16_t stride; uint8_t ra, rb; };")
+    lines.append("// wa, wb: the largest sum of |coefficients| of operand A / B over the busy lanes")
+    lines.append("struct SRound { uint32_t desc; uint16_t stride; uint8_t ra, rb, wa, wb; };")
     lines.append("constexpr int MIXED = %d, PAIR = %d;" % (MIXED, PAIR))
     lines.append("// the op and round tables sit in the constant address space, so that loads at")
     lines.append("// uniform indices are scalar loads")

@@ -276,8 +276,10 @@ __device__ __forceinline__ int term_at(const u32 (&rec)[RW], int k, const Region
   }
 }
 
-// the sum of the N >= 1 terms at halfwords FIRST.. of the record, carried
-template <int FIRST, int N, int REG, int RW>
+// the sum of the N >= 1 terms at halfwords FIRST.. of the record, carried (CARRY), or,
+// for a sum of weight <= 2 of normalized slots (limbs below 2^29, a valid multiplier
+// operand as it is), the low words of the limb sums
+template <int FIRST, int N, int REG, int RW, bool CARRY = true>
 __device__ __forceinline__ Fp sum_terms(const u32 (&rec)[RW], const Regions& R) {
   static_assert(N >= 1 && FIRST + N <= 2 * RW, "terms of a busy round");
   i64 acc[NL];
@@ -302,7 +304,14 @@ __device__ __forceinline__ Fp sum_terms(const u32 (&rec)[RW], const Regions& R) 
       }
     }
   }
-  return carry64(acc);
+  if constexpr (CARRY) {
+    return carry64(acc);
+  } else {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < NL; i++) r.l[i] = (i32)acc[i];
+    return r;
+  }
 }
 // a one-term operand with coefficient 1 on every busy lane
 template <int AT, int REG, int RW>
@@ -319,7 +328,7 @@ struct RoundC {
   // records in DESC_S (region-relative terms, lane-major) or in the interpreter's DESC
   static constexpr sched::SRound s0 = sched::Tab<G>::SROUNDS_C[o.first + RI];
   static constexpr bool S = s0.desc != 0xffffffffu;
-  static constexpr sched::SRound sr = S ? s0 : sched::SRound{rd.desc, rd.q, sched::MIXED, sched::MIXED};
+  static constexpr sched::SRound sr = S ? s0 : sched::SRound{rd.desc, rd.q, sched::MIXED, sched::MIXED, 255, 255};
   static constexpr int NR = o.np + o.no, Q = rd.q, RW = 4 * Q;
   // whether the round reads halfword k of the record: dst, then its A and B terms
   // (product rounds) or its terms (output rounds)
@@ -355,9 +364,9 @@ __device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, b
     if (work) {
       Fp a, b;
       if constexpr (rd.flags & 1) a = load_term<1, C::sr.ra>(rec, R);
-      else a = sum_terms<1, rd.na, C::sr.ra>(rec, R);
+      else a = sum_terms<1, rd.na, C::sr.ra, C::RW, (C::sr.wa > 2)>(rec, R);
       if constexpr (rd.flags & 2) b = load_term<sched::B_AT, C::sr.rb>(rec, R);
-      else b = sum_terms<sched::B_AT, rd.nb, C::sr.rb>(rec, R);
+      else b = sum_terms<sched::B_AT, rd.nb, C::sr.rb, C::RW, (C::sr.wb > 2)>(rec, R);
       const Fp pr = mul(a, b);
       if (active && dst != sched::IDLE) lds_st(R.t + (int)dst * SLOT, pr);
     }

@@ -257,10 +257,24 @@ def test_static_tables_execute_the_straight_line_ops(variants, G):
                     r, idx = reg, off // 14
                 return 0 if coef == 0 else coef * regions[r][idx]
 
+            def weight(ts, reg):
+                w = 0
+                for t in ts:
+                    if reg == GSCH.MIXED:
+                        c = decode(t)[2]
+                    elif reg >= GSCH.PAIR:
+                        c = (t >> 11) & 31
+                        c -= 32 if c >= 16 else 0
+                    else:
+                        c = (t >> 10) & 63
+                        c -= 64 if c >= 32 else 0
+                    w += abs(c)
+                return w
+
             out = {}
             for k in range(np_ + no):
                 desc, na, nb, q, flags = rounds_c[first + k]
-                sdesc, stride, ra, rb = srounds[first + k]
+                sdesc, stride, ra, rb, wa, wb = srounds[first + k]
                 for lane in range(G):
                     at = 4 * (sdesc + lane * stride)
                     hw = [h for w in words[at:at + 4 * q] for h in (w & 0xFFFF, w >> 16)]
@@ -269,6 +283,8 @@ def test_static_tables_execute_the_straight_line_ops(variants, G):
                     if k < np_:
                         x = sum(term(t, ra) for t in hw[1:1 + na])
                         y = sum(term(t, rb) for t in hw[GSCH.B_AT:GSCH.B_AT + nb])
+                        # the weights the kernel relies on (<= 2: the operand is not carried)
+                        assert weight(hw[1:1 + na], ra) <= wa and weight(hw[GSCH.B_AT:GSCH.B_AT + nb], rb) <= wb
                         regions[T][hw[0]] = x * y % P
                     else:
                         out[hw[0]] = sum(term(t, ra) for t in hw[1:1 + na]) % P
