@@ -41,6 +41,7 @@ EXTENSION_SYMBOLS = (
     "blsgpu_set_profiling",
     "blsgpu_phase_times",
     "blsgpu_register_keys_affine",
+    "blsgpu_register_keys_compressed",
     "blsgpu_verify_batches",
     "blsgpu_verify_batches_device",
     "blsgpu_set_batched_engine",
@@ -104,6 +105,7 @@ def load(path=None):
         "blsgpu_set_profiling": ([i32], i32),
         "blsgpu_phase_times": ([vp, vp], i32),
         "blsgpu_register_keys_affine": ([vp, i32, vp], i32),
+        "blsgpu_register_keys_compressed": ([vp, i32, vp, vp, i32], i32),
         "blsgpu_verify_batches": ([vp, vp, vp, vp, i32, vp, vp], i32),
         "blsgpu_verify_batches_device": ([vp, vp, vp, vp, i32, vp, vp, vp], i32),
         "blsgpu_set_batched_engine": ([i32, i32, i32, i32, i32], i32),
@@ -215,6 +217,19 @@ def register_keys_affine(pks):
     if load().blsgpu_register_keys_affine(_p(pks), pks.shape[0], _p(h)) < 0:
         raise RuntimeError("register_keys_affine: " + last_error())
     return h
+
+
+def register_keys_compressed(pks96, check_subgroup=True):
+    """Decompresses, validates and registers n compressed G2 keys ((n, 96) u8) on the
+    GPU.  Returns (handles, status): handle -1 and status 1 (identity) or a negative
+    decode code for keys that were not registered."""
+    pks96 = np.ascontiguousarray(pks96, dtype=np.uint8).reshape(-1, 96)
+    n = pks96.shape[0]
+    h = np.zeros(n, dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    if load().blsgpu_register_keys_compressed(_p(pks96), n, _p(h), _p(st), int(bool(check_subgroup))) < 0:
+        raise RuntimeError("register_keys_compressed: " + last_error())
+    return h, st
 
 
 def verify_batches(batches, want_gt=False):

@@ -258,6 +258,57 @@ int blsgpu_register_keys_affine(const uint64_t* pks, int n, int32_t* handles) {
   return first;
 }
 
+int blsgpu_register_keys_compressed(const uint8_t* pks96, int n, int32_t* handles, int32_t* status,
+                                    int check_subgroup) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (!pks96 || n <= 0) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  CHK(init_locked());
+  CHK(claim_workspace_locked(g.s));
+  const size_t row = 24 * 8;
+  CHK(g.comp.ensure((size_t)n * 96));
+  CHK(g.pk_ref.ensure((size_t)n * row * 2));  // decoded points, then the accepted ones packed
+  CHK(g.status.ensure((size_t)n * 4));
+  HIPCHK(hipMemcpyAsync(g.comp.p, pks96, (size_t)n * 96, hipMemcpyHostToDevice, g.s));
+  CHK(bls::launch_g2_decompress(g.comp.as<uint8_t>(), g.pk_ref.as<uint64_t>(), g.status.as<int32_t>(), n,
+                                check_subgroup, g.s));
+  std::vector<int32_t> st(n);
+  HIPCHK(hipMemcpyAsync(st.data(), g.status.p, (size_t)n * 4, hipMemcpyDeviceToHost, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  // accepted keys (status 0: decoded, not the identity) packed in order, one copy per run
+  uint8_t* dec = g.pk_ref.as<uint8_t>();
+  uint8_t* packed = dec + (size_t)n * row;
+  int k = 0;
+  for (int i = 0; i < n;) {
+    if (st[i] != 0) {
+      i++;
+      continue;
+    }
+    int j = i;
+    while (j < n && st[j] == 0) j++;
+    HIPCHK(hipMemcpyAsync(packed + (size_t)k * row, dec + (size_t)i * row, (size_t)(j - i) * row,
+                          hipMemcpyDeviceToDevice, g.s));
+    k += j - i;
+    i = j;
+  }
+  const int first = g.nkeys;
+  if (k > 0) {
+    CHK(grow_keys_locked(k));
+    CHK(bls::launch_g2_prepare(reinterpret_cast<const uint64_t*>(packed),
+                               g.keys.as<int32_t>() + (size_t)g.nkeys * bls::KEY_TABLE_WORDS, k, g.s));
+    HIPCHK(hipStreamSynchronize(g.s));
+    g.nkeys += k;
+  }
+  for (int i = 0, h = first; i < n; i++) {
+    if (status) status[i] = st[i];
+    if (handles) handles[i] = st[i] == 0 ? h++ : -1;
+  }
+  return k;
+}
+
 int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]) {
   std::lock_guard<std::mutex> lk(g.mu);
   g_err.clear();

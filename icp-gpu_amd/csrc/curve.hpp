@@ -38,6 +38,12 @@ struct FpCalls {
   static __device__ __forceinline__ Fp mul(const Fp& a, const Fp& b) { return mul_call(a, b); }
   static __device__ __forceinline__ Fp sqr(const Fp& a) { return sqr_call(a); }
 };
+static __device__ __noinline__ Fp2 mul2_call(Fp2 a, Fp2 b) { return mul(a, b); }
+static __device__ __noinline__ Fp2 sqr2_call(Fp2 a) { return sqr(a); }
+struct Fp2Calls {
+  static __device__ __forceinline__ Fp2 mul(const Fp2& a, const Fp2& b) { return mul2_call(a, b); }
+  static __device__ __forceinline__ Fp2 sqr(const Fp2& a) { return sqr2_call(a); }
+};
 
 // dbl-2009-l (a = 0).  Z = 0 (identity) maps to Z3 = 0.
 template <class F, class M = FieldInline>
@@ -54,35 +60,35 @@ __device__ __forceinline__ Jac<F> jdbl(const Jac<F>& p) {
   return r;
 }
 // madd-2007-bl: p Jacobian, q affine (not the identity).
-template <class F>
+template <class F, class M = FieldInline>
 __device__ __forceinline__ Jac<F> jadd_aff(const Jac<F>& p, const Aff<F>& q, const F& one) {
   if (is_zero(p.z)) return {q.x, q.y, one};
-  F Z1Z1 = sqr_any(p.z);
-  F U2 = mul(q.x, Z1Z1);
-  F S2 = mul(q.y, mul(p.z, Z1Z1));
+  F Z1Z1 = M::sqr(p.z);
+  F U2 = M::mul(q.x, Z1Z1);
+  F S2 = M::mul(q.y, M::mul(p.z, Z1Z1));
   F H = norm(sub(U2, p.x));
   F R = norm(dbl(sub(S2, p.y)));
   if (is_zero(H)) {
-    if (is_zero(R)) return jdbl(p);
+    if (is_zero(R)) return jdbl<F, M>(p);
     Jac<F> o;
     o.x = one; o.y = one; o.z = norm(sub(one, one));
     return o;
   }
-  F HH = sqr_any(H);
+  F HH = M::sqr(H);
   F I = norm(dbl(dbl(HH)));
-  F J = mul(H, I);
-  F V = mul(p.x, I);
+  F J = M::mul(H, I);
+  F V = M::mul(p.x, I);
   Jac<F> r;
-  r.x = norm(sub(sub(sub(sqr_any(R), J), V), V));
-  r.y = norm(sub(mul(R, sub(V, r.x)), dbl(mul(p.y, J))));
-  r.z = norm(sub(sub(sqr_any(carry(add(p.z, H))), Z1Z1), HH));
+  r.x = norm(sub(sub(sub(M::sqr(R), J), V), V));
+  r.y = norm(sub(M::mul(R, sub(V, r.x)), dbl(M::mul(p.y, J))));
+  r.z = norm(sub(sub(M::sqr(carry(add(p.z, H))), Z1Z1), HH));
   return r;
 }
-template <class F>
+template <class F, class M = FieldInline>
 __device__ __forceinline__ bool jeq_aff(const Jac<F>& p, const Aff<F>& q) {
   if (is_zero(p.z)) return false;
-  F z2 = sqr_any(p.z);
-  return eq(p.x, mul(q.x, z2)) && eq(p.y, mul(q.y, mul(z2, p.z)));
+  F z2 = M::sqr(p.z);
+  return eq(p.x, M::mul(q.x, z2)) && eq(p.y, M::mul(q.y, M::mul(z2, p.z)));
 }
 template <class F>
 __device__ __forceinline__ Aff<F> to_affine(const Jac<F>& p) {
@@ -121,6 +127,60 @@ __device__ __forceinline__ bool g1_in_subgroup(const Aff<Fp>& p) {
   Jac<Fp> b = g1_mul_absx(a1);
   Aff<Fp> phi = {mul(p.x, fp_const(BETA_M)), neg(p.y)};
   return jeq_aff(b, phi);
+}
+
+// ---- G2: membership and decompression (public keys) ----
+// a^e for a fixed public exponent (12 x 32-bit words, little-endian), in Fp2
+__device__ __forceinline__ Fp2 pow2_fixed(const Fp2& a, const u32 (&e)[12]) {
+  Fp2 r = {fp_one(), fp_zero()}, b = a;
+  for (int w = 0; w < 12; w++) {
+    u32 bits = e[w];
+    for (int i = 0; i < 32; i++) {
+      if (bits & 1) r = mul(r, b);
+      b = sqr(b);
+      bits >>= 1;
+    }
+  }
+  return r;
+}
+// Square root in Fp2 for p = 3 mod 4 (Adj and Rodriguez-Henriquez, eprint 2012/685,
+// Algorithm 9): a1 = a^((p-3)/4), alpha = a1^2 a, x0 = a1 a; alpha == -1 gives u x0,
+// otherwise (1 + alpha)^((p-1)/2) x0.  ok = (the root squares back to a).
+__device__ __forceinline__ Fp2 sqrt2(const Fp2& a, bool& ok) {
+  if (is_zero(a)) {
+    ok = true;
+    return a;
+  }
+  const Fp2 a1 = pow2_fixed(a, P34_EXP);
+  const Fp2 alpha = mul(sqr(a1), a), x0 = mul(a1, a);
+  const Fp2 m1 = {neg(fp_one()), fp_zero()};
+  Fp2 r;
+  if (eq(alpha, m1)) r = {neg(x0.c1), x0.c0};
+  else r = mul(pow2_fixed(norm(add(alpha, Fp2{fp_one(), fp_zero()})), HALF_P32), x0);
+  ok = eq(sqr(r), a);
+  return r;
+}
+// y lexicographically largest in Fp2 (the sort flag): y.c1 largest, or y.c1 == 0 and
+// y.c0 largest
+__device__ __forceinline__ bool lex_largest2(const Fp2& a) {
+  return lex_largest(a.c1) || (is_zero(a.c1) && lex_largest(a.c0));
+}
+// psi(x, y) = (conj(x) cx, conj(y) cy), the untwist-Frobenius-twist endomorphism
+__device__ __forceinline__ Aff<Fp2> g2_psi(const Aff<Fp2>& q) {
+  const Fp2 cx = {fp_const(PSI_X0), fp_const(PSI_X1)}, cy = {fp_const(PSI_Y0), fp_const(PSI_Y1)};
+  return {mul(conj(q.x), cx), mul(conj(q.y), cy)};
+}
+// G2 membership of a point on the twist: psi(Q) == [x]Q = -[|x|]Q (eprint 2021/1130
+// sec. 4, eprint 2022/352; ic_bls12_381's G2Affine::is_torsion_free)
+__device__ __forceinline__ bool g2_in_subgroup(const Aff<Fp2>& q) {
+  const Fp2 one = {fp_one(), fp_zero()};
+  Jac<Fp2> acc = {q.x, q.y, one};
+  for (int i = 62; i >= 0; i--) {
+    acc = jdbl<Fp2, Fp2Calls>(acc);
+    if ((BLS_X >> i) & 1) acc = jadd_aff<Fp2, Fp2Calls>(acc, q, one);
+  }
+  acc.y = neg(acc.y);
+  return jeq_aff<Fp2, Fp2Calls>(acc, g2_psi(q));
 }
 
 // ---- G1 decompression (ZCash encoding: bit7 compressed, bit6 infinity, bit5 sort) ----
@@ -237,6 +297,54 @@ __device__ __forceinline__ int32_t g1_decompress(const u32 be[12], Aff<Fp>& out,
   return DEC_OK;
 }
 // 48-byte compressed encoding of an affine point (big-endian words as above)
+// 96 bytes given as 24 big-endian-packed words: x.c1 (words 0..11, flags in the top
+// three bits) then x.c0.  Semantics of ic_bls12_381 G2Affine::from_compressed (the
+// reference has no G2 decompression; its callers pass prepared lines):
+// check_subgroup=false gives from_compressed_unchecked.
+__device__ __forceinline__ int32_t g2_decompress(const u32 be[24], Aff<Fp2>& out, bool check_subgroup) {
+  const u32 top = be[0] >> 24;
+  const bool cflag = top & 0x80, iflag = top & 0x40, sflag = top & 0x20;
+  u32 w1[12], w0[12];  // little-endian 32-bit words of x.c1 and x.c0
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    w1[j] = be[11 - j];
+    w0[j] = be[23 - j];
+  }
+  w1[11] &= 0x1fffffffu;
+  out.x = {fp_zero(), fp_zero()};
+  out.y = {fp_zero(), fp_zero()};
+  if (!cflag) return DEC_BAD_FLAGS;
+  if (iflag) {
+    u32 acc = 0;
+#pragma unroll
+    for (int j = 0; j < 12; j++) acc |= w1[j] | w0[j];
+    return (!sflag && acc == 0) ? DEC_IDENTITY : DEC_BAD_FLAGS;
+  }
+  auto below_p = [](const u32 (&w)[12]) {
+    bool lt = false, decided = false;
+#pragma unroll
+    for (int j = 11; j >= 0; j--) {
+      if (!decided && w[j] != P32[j]) {
+        lt = w[j] < P32[j];
+        decided = true;
+      }
+    }
+    return lt;
+  };
+  if (!below_p(w1) || !below_p(w0)) return DEC_X_NOT_CANONICAL;
+  const Fp2 x = {from_plain(unpack384(w0)), from_plain(unpack384(w1))};
+  const Fp b4 = fp_const(B_G2_M);
+  const Fp2 y2 = norm(add(mul(sqr(x), x), Fp2{b4, b4}));
+  bool ok;
+  Fp2 y = sqrt2(y2, ok);
+  if (!ok) return DEC_NOT_ON_CURVE;
+  if (lex_largest2(y) != sflag) y = neg(y);
+  out.x = x;
+  out.y = norm(y);
+  if (check_subgroup && !g2_in_subgroup(out)) return DEC_NOT_IN_SUBGROUP;
+  return DEC_OK;
+}
+
 __device__ __forceinline__ void g1_compress(const Aff<Fp>& p, bool identity, u32 be[12]) {
   if (identity) {
 #pragma unroll

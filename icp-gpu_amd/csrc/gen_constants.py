@@ -132,6 +132,15 @@ def main():
     arr("FROB12_1", to_int(g12[1]), "(1+u)^((p-1)/6) = (., this)")
     arr("BETA_M", to_int(beta), "cube root of unity: (beta x, y) = [-x^2](x, y) on G1")
     arr("B_G2_M", to_int(4), "G2 curve constant b' = 4(1+u) = (this, this)")
+    # psi(x, y) = (x^p cx, y^p cy) on the twist, cx = (1+u)^-((p-1)/3), cy = (1+u)^-((p-1)/2)
+    # (the endomorphism of the G2 membership test psi(Q) == [x]Q, eprint 2021/1130 sec. 4)
+    cx = f2pow(f2pow((1, 1), (P - 1) // 3), P * P - 2)
+    cy = f2pow(f2pow((1, 1), (P - 1) // 2), P * P - 2)
+    assert f2mul(cx, f2pow((1, 1), (P - 1) // 3)) == (1, 0)
+    arr("PSI_X0", to_int(cx[0]), "psi: cx.c0")
+    arr("PSI_X1", to_int(cx[1]), "psi: cx.c1")
+    arr("PSI_Y0", to_int(cy[0]), "psi: cy.c0")
+    arr("PSI_Y1", to_int(cy[1]), "psi: cy.c1")
     # exponents as 32-bit words, little-endian
     def words(v, n):
         return ", ".join("0x%08xu" % ((v >> (32 * i)) & 0xffffffff) for i in range(n))
@@ -139,6 +148,8 @@ def main():
     emit("constexpr uint32_t SQRT_EXP[12] = {%s};" % words((P + 1) // 4, 12))
     emit("constexpr uint32_t INV_EXP[12] = {%s};" % words(P - 2, 12))
     emit("constexpr uint32_t HALF_P32[12] = {%s};" % words((P - 1) // 2, 12))
+    emit("// (p-3)/4 for the square root in Fp2")
+    emit("constexpr uint32_t P34_EXP[12] = {%s};" % words((P - 3) // 4, 12))
     emit("constexpr uint32_t P32[12] = {%s};" % words(P, 12))
     emit("constexpr uint32_t R_ORDER32[8] = {%s};" % words(R_ORDER, 8))
     emit("constexpr uint64_t BLS_X = 0x%016xull;  // x = -BLS_X" % BLS_X)

@@ -14,6 +14,9 @@ int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t 
 int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, int soa, hipStream_t s);
 int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s);
 int launch_g2_prepare(const uint64_t* q, int32_t* out, int K, hipStream_t s);
+// K compressed G2 points (96 bytes each) -> reference-form affine points + decode status
+int launch_g2_decompress(const uint8_t* in, uint64_t* q_ref, int32_t* status, int K, int check_subgroup,
+                         hipStream_t s);
 int launch_g1_decompress(const uint8_t* in, int32_t* pts, int32_t* status, int M, int check_subgroup,
                          hipStream_t s);
 int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* out, int M, hipStream_t s);

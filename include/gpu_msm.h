@@ -81,6 +81,15 @@ int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]);
 /* n keys from affine points (n x 24 u64) in one launch; handles[i] (nullable) receives
  * the handle of key i.  Returns the first handle, or -1. */
 int blsgpu_register_keys_affine(const uint64_t* pks, int n, int32_t* handles);
+/* n keys from compressed G2 points (n x 96 bytes: x.c1 then x.c0, big-endian, flags in
+ * the top three bits of byte 0), decompressed, validated and prepared on the GPU with
+ * ic_bls12_381's G2Affine::from_compressed semantics (check_subgroup = 0 gives
+ * from_compressed_unchecked).  status[i] (nullable): 0 accepted, 1 the identity, or a
+ * negative BLSGPU_ERR_* code; handles[i] (nullable): the key's handle, -1 unless status
+ * is 0 (the identity is not registered).  Accepted keys get consecutive handles in input
+ * order.  Returns the number of keys registered, or -1. */
+int blsgpu_register_keys_compressed(const uint8_t* pks96, int n, int32_t* handles, int32_t* status,
+                                    int check_subgroup);
 /* copies a registered key's lines back out (2448 u64, reference form) */
 int blsgpu_key_coeffs(int handle, uint64_t coeffs[2448]);
 int blsgpu_num_keys(void);

@@ -922,6 +922,110 @@ void orc_g1_decompress_legacy(const uint8_t in[48], u64 out[12]) {
     if (sflag != fp_lex_largest(&a.y)) fp_neg(&a.y, &a.y);
     ST(out, a);
 }
+/* ---- G2 compression (ic_bls12_381 G2Affine::to_compressed / from_compressed) ----
+ * 96 bytes: x.c1 then x.c0, each 48 big-endian bytes; the flags sit in the top three
+ * bits of byte 0 as for G1 (compression, infinity, sort).  The sort flag is set when y is
+ * lexicographically largest in Fp2: y.c1 lexicographically largest, or y.c1 == 0 and
+ * y.c0 lexicographically largest. */
+static int fp2_lex_largest(const fp2* a) {
+    return fp_lex_largest(&a->c1) || (fp_is_zero(&a->c1) && fp_lex_largest(&a->c0));
+}
+/* Square root in Fp2 for p = 3 mod 4 (Adj and Rodriguez-Henriquez, eprint 2012/685,
+ * Algorithm 9): a1 = a^((p-3)/4), alpha = a1^2 a, x0 = a1 a; alpha == -1 gives the root
+ * u x0, otherwise (1 + alpha)^((p-1)/2) x0.  Returns 1 when the candidate squares back
+ * to a. */
+static int fp2_sqrt(fp2* r, const fp2* a) {
+    static const u64 P34[6] = {0xee7fbfffffffeaaaULL, 0x07aaffffac54ffffULL, 0xd9cc34a83dac3d89ULL,
+                               0xd91dd2e13ce144afULL, 0x92c6e9ed90d2eb35ULL, 0x0680447a8e5ff9a6ULL};
+    static const u64 P12[6] = {0xdcff7fffffffd555ULL, 0x0f55ffff58a9ffffULL, 0xb39869507b587b12ULL,
+                               0xb23ba5c279c2895fULL, 0x258dd3db21a5d66bULL, 0x0d0088f51cbff34dULL};
+    if (fp2_is_zero(a)) { *r = *a; return 1; }
+    fp2 a1 = fp2_one(), base = *a;
+    for (int w = 0; w < 6; w++)
+        for (int b = 0; b < 64; b++) {
+            if ((P34[w] >> b) & 1) fp2_mul(&a1, &a1, &base);
+            fp2_sqr(&base, &base);
+        }
+    fp2 alpha, x0, m1, cand;
+    fp2_sqr(&alpha, &a1);
+    fp2_mul(&alpha, &alpha, a);
+    fp2_mul(&x0, &a1, a);
+    m1 = fp2_one();
+    fp2_neg(&m1, &m1);
+    if (fp2_eq(&alpha, &m1)) {
+        fp nc1; fp_neg(&nc1, &x0.c1);
+        cand.c0 = nc1; cand.c1 = x0.c0;
+    } else {
+        fp2 b = fp2_one(), t;
+        fp2_add(&t, &alpha, &b);
+        base = t;
+        for (int w = 0; w < 6; w++)
+            for (int bit = 0; bit < 64; bit++) {
+                if ((P12[w] >> bit) & 1) fp2_mul(&b, &b, &base);
+                fp2_sqr(&base, &base);
+            }
+        fp2_mul(&cand, &b, &x0);
+    }
+    fp2 c2; fp2_sqr(&c2, &cand);
+    *r = cand;
+    return fp2_eq(&c2, a);
+}
+void orc_g2_compress(const u64 q[24], int inf, uint8_t out[96]) {
+    if (inf) { memset(out, 0, 96); out[0] = 0xc0; return; }
+    LD(g2a, a, q);
+    fp_to_be(out, &a.x.c1);
+    fp_to_be(out + 48, &a.x.c0);
+    out[0] |= 0x80;
+    if (fp2_lex_largest(&a.y)) out[0] |= 0x20;
+}
+/* G2Affine::from_compressed_unchecked: the same flag rules as G1; both coordinates of x
+ * canonical; y^2 = x^3 + 4(1 + u) solvable; y chosen by the sort flag. */
+int orc_g2_decompress_unchecked(const uint8_t in[96], u64 out[24]) {
+    int cflag = (in[0] >> 7) & 1, iflag = (in[0] >> 6) & 1, sflag = (in[0] >> 5) & 1;
+    uint8_t xb[96];
+    memcpy(xb, in, 96);
+    xb[0] &= 0x1f;
+    fp r1, r0;
+    fp_from_be_raw(&r1, xb);
+    fp_from_be_raw(&r0, xb + 48);
+    memset(out, 0, 192);
+    if (!cflag) return ORC_DEC_BAD_FLAGS;
+    if (iflag) {
+        if (sflag || !fp_is_zero(&r1) || !fp_is_zero(&r0)) return ORC_DEC_BAD_FLAGS;
+        return ORC_DEC_IDENTITY;
+    }
+    if (!fp_lt_p(&r1) || !fp_lt_p(&r0)) return ORC_DEC_X_NOT_CANONICAL;
+    g2a a;
+    fp_to_mont(&a.x.c1, &r1);
+    fp_to_mont(&a.x.c0, &r0);
+    fp2 y2, b = fp2_one();
+    fp_add(&b.c0, &b.c0, &b.c0); fp_add(&b.c0, &b.c0, &b.c0);  /* 4 */
+    b.c1 = b.c0;                                                   /* 4 + 4u */
+    fp2_sqr(&y2, &a.x);
+    fp2_mul(&y2, &y2, &a.x);
+    fp2_add(&y2, &y2, &b);
+    if (!fp2_sqrt(&a.y, &y2)) return ORC_DEC_NOT_ON_CURVE;
+    if (fp2_lex_largest(&a.y) != sflag) fp2_neg(&a.y, &a.y);
+    ST(out, a);
+    return ORC_DEC_OK;
+}
+int orc_g2_decompress(const uint8_t in[96], u64 out[24]) {
+    int rc = orc_g2_decompress_unchecked(in, out);
+    if (rc != ORC_DEC_OK) return rc;
+    LD(g2a, a, out);
+    if (!g2a_in_subgroup(&a)) return ORC_DEC_NOT_IN_SUBGROUP;
+    return ORC_DEC_OK;
+}
+int orc_g2_is_on_curve(const u64 q[24]) {
+    LD(g2a, a, q);
+    fp2 y2, x3, b = fp2_one();
+    fp_add(&b.c0, &b.c0, &b.c0); fp_add(&b.c0, &b.c0, &b.c0);
+    b.c1 = b.c0;
+    fp2_sqr(&y2, &a.y);
+    fp2_sqr(&x3, &a.x); fp2_mul(&x3, &x3, &a.x);
+    fp2_add(&x3, &x3, &b);
+    return fp2_eq(&y2, &x3);
+}
 int orc_g1_is_on_curve(const u64 p[12]) { LD(g1a, a, p); return g1a_on_curve(&a); }
 int orc_g1_in_subgroup(const u64 p[12]) { LD(g1a, a, p); return g1a_in_subgroup(&a); }
 int orc_g2_in_subgroup(const u64 q[24]) { LD(g2a, a, q); return g2a_in_subgroup(&a); }

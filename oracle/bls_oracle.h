@@ -104,6 +104,12 @@ int  orc_g1_in_subgroup(const uint64_t p[12]);
 int  orc_g1_mul(const uint64_t p[12], const uint8_t k[32], uint64_t out[12]);
 int  orc_g2_mul(const uint64_t q[24], const uint8_t k[32], uint64_t out[24]);
 int  orc_g2_in_subgroup(const uint64_t q[24]);
+/* G2 compression as ic_bls12_381 (96 bytes: x.c1 then x.c0 big-endian, flags in byte
+ * 0); decompress returns an ORC_DEC_* code, the checked form also [r]Q == O */
+void orc_g2_compress(const uint64_t q[24], int inf, uint8_t out[96]);
+int  orc_g2_decompress(const uint8_t in[96], uint64_t out[24]);
+int  orc_g2_decompress_unchecked(const uint8_t in[96], uint64_t out[24]);
+int  orc_g2_is_on_curve(const uint64_t q[24]);
 void orc_g1_neg(const uint64_t p[12], uint64_t out[12]);
 void orc_g1_generator(uint64_t out[12]);
 void orc_g2_generator(uint64_t out[24]);

@@ -115,6 +115,42 @@ def g1_decompress(b, check=True):
     return rc, o
 
 
+def g2_compress(q, inf=False):
+    o = np.zeros(96, np.uint8)
+    lib().orc_g2_compress(ptr(np.ascontiguousarray(q)), int(inf), ptr(o))
+    return o
+
+
+def g2_decompress(b, check=True):
+    o = np.zeros(24, np.uint64)
+    f = lib().orc_g2_decompress if check else lib().orc_g2_decompress_unchecked
+    rc = f(ptr(np.ascontiguousarray(b, dtype=np.uint8)), ptr(o))
+    return rc, o
+
+
+def g2_is_on_curve(q):
+    return bool(lib().orc_g2_is_on_curve(ptr(np.ascontiguousarray(q))))
+
+
+def g2_in_subgroup(q):
+    return bool(lib().orc_g2_in_subgroup(ptr(np.ascontiguousarray(q))))
+
+
+def g2_off_subgroup_point(seed, idx):
+    """an on-curve G2 point outside the r-subgroup: x from seeded bytes until
+    x^3 + 4(1 + u) is a square (the cofactor is ~2^381, so such a point misses G2)"""
+    k = 0
+    while True:
+        xb = bytearray(rand_bytes(seed, 3_000_000 + 1000 * idx + k, 96))
+        xb[0] &= 0x0F
+        xb[48] &= 0x0F
+        xb[0] |= 0x80
+        rc, q = g2_decompress(np.frombuffer(bytes(xb), np.uint8), check=False)
+        if rc == 0 and not g2_in_subgroup(q):
+            return bytes(xb), q
+        k += 1
+
+
 def hash_to_g1(msg):
     m = np.frombuffer(bytes(msg), dtype=np.uint8).copy()
     o = np.zeros(12, np.uint64)
