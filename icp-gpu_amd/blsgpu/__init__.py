@@ -42,6 +42,8 @@ EXTENSION_SYMBOLS = (
     "blsgpu_phase_times",
     "blsgpu_register_keys_affine",
     "blsgpu_register_keys_compressed",
+    "blsgpu_hash_to_g1",
+    "blsgpu_verify_messages",
     "blsgpu_verify_batches",
     "blsgpu_verify_batches_device",
     "blsgpu_set_batched_engine",
@@ -106,6 +108,8 @@ def load(path=None):
         "blsgpu_phase_times": ([vp, vp], i32),
         "blsgpu_register_keys_affine": ([vp, i32, vp], i32),
         "blsgpu_register_keys_compressed": ([vp, i32, vp, vp, i32], i32),
+        "blsgpu_hash_to_g1": ([vp, vp, i32, vp, i32, vp, vp, i32], i32),
+        "blsgpu_verify_messages": ([vp, vp, vp, vp, i32, vp, vp, vp, i32], i32),
         "blsgpu_verify_batches": ([vp, vp, vp, vp, i32, vp, vp], i32),
         "blsgpu_verify_batches_device": ([vp, vp, vp, vp, i32, vp, vp, vp], i32),
         "blsgpu_set_batched_engine": ([i32, i32, i32, i32, i32], i32),
@@ -230,6 +234,42 @@ def register_keys_compressed(pks96, check_subgroup=True):
     if load().blsgpu_register_keys_compressed(_p(pks96), n, _p(h), _p(st), int(bool(check_subgroup))) < 0:
         raise RuntimeError("register_keys_compressed: " + last_error())
     return h, st
+
+
+def _pack_messages(msgs):
+    offs = np.zeros(len(msgs) + 1, dtype=np.int64)
+    offs[1:] = np.cumsum([len(m) for m in msgs])
+    buf = np.frombuffer(b"".join(bytes(m) for m in msgs) or b"\0", dtype=np.uint8).copy()
+    return buf, offs
+
+
+def hash_to_g1(msgs, dst, negate=False):
+    """H(m) for each message (RFC 9380 BLS12381G1_XMD:SHA-256_SSWU_RO_ with `dst`) on the
+    GPU: (n, 12) u64 affine points in the boundary form, and the identity flags."""
+    buf, offs = _pack_messages(msgs)
+    d = np.frombuffer(bytes(dst), dtype=np.uint8).copy()
+    n = len(msgs)
+    out = np.zeros((n, 12), dtype=np.uint64)
+    ident = np.zeros(n, dtype=np.int32)
+    _check(load().blsgpu_hash_to_g1(_p(buf), _p(offs), n, _p(d), len(dst), _p(out), _p(ident), int(bool(negate))),
+           "blsgpu_hash_to_g1")
+    return out, ident
+
+
+def verify_messages(sigs48, msgs, dst, key_idx, want_gt=False):
+    """Fused wire-format path: compressed signatures, messages (hashed on the GPU) and key
+    handles; results as verify_compressed."""
+    sigs48 = np.ascontiguousarray(sigs48, dtype=np.uint8).reshape(-1, 48)
+    key_idx = np.ascontiguousarray(key_idx, dtype=np.int32)
+    buf, offs = _pack_messages(msgs)
+    d = np.frombuffer(bytes(dst), dtype=np.uint8).copy()
+    m = sigs48.shape[0]
+    assert len(msgs) == m and key_idx.shape[0] == m
+    res = np.zeros(m, dtype=np.int32)
+    gt = np.zeros((m, 72), dtype=np.uint64) if want_gt else None
+    _check(load().blsgpu_verify_messages(_p(sigs48), _p(buf), _p(offs), _p(d), len(dst), _p(key_idx), _p(res),
+                                         _p(gt), m), "blsgpu_verify_messages")
+    return (res, gt) if want_gt else res
 
 
 def verify_batches(batches, want_gt=False):

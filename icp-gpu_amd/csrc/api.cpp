@@ -64,6 +64,7 @@ int init_locked() {
   HIPCHK(hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g.ev_h2c, hipEventDisableTiming));
   // generator lines, once
   CHK(g.g2_ref.ensure(24 * 8));
   CHK(g.gen_lines.ensure(bls::KEY_TABLE_WORDS * 4));
@@ -347,14 +348,59 @@ void blsgpu_reset_keys(void) {
   g.nkeys = 0;
 }
 
-int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, const int32_t* key_idx,
-                             int32_t* results, uint64_t* gt_out, int M) {
+// messages (bytes msgs[offsets[i] .. offsets[i+1]) of n) and the DST, copied to the
+// device on g.s; checks the offsets and the DST length (1..255, RFC 9380 sec. 5.3.1)
+static int upload_messages_locked(const uint8_t* msgs, const int64_t* offsets, int n, const uint8_t* dst,
+                                  int dst_len) {
+  if (!offsets || !dst || dst_len < 1 || dst_len > 255 || offsets[0] < 0) {
+    g_err = "invalid message arguments";
+    return -1;
+  }
+  for (int i = 0; i < n; i++)
+    if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > (1 << 24)) {
+      g_err = "message offsets must be non-decreasing, messages at most 16 MiB";
+      return -1;
+    }
+  const size_t total = (size_t)offsets[n];
+  if (total > 0 && !msgs) {
+    g_err = "invalid message arguments";
+    return -1;
+  }
+  CHK(g.msg_bytes.ensure(total > 0 ? total : 1));
+  CHK(g.msg_offs.ensure((size_t)(n + 1) * 8));
+  CHK(g.dst_bytes.ensure(256));
+  if (total > 0) HIPCHK(hipMemcpyAsync(g.msg_bytes.p, msgs, total, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.msg_offs.p, offsets, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, g.s));
+  HIPCHK(hipMemcpyAsync(g.dst_bytes.p, dst, (size_t)dst_len, hipMemcpyHostToDevice, g.s));
+  return 0;
+}
+
+int blsgpu_hash_to_g1(const uint8_t* msgs, const int64_t* offsets, int n, const uint8_t* dst, int dst_len,
+                      uint64_t* out, int32_t* is_identity, int negate) {
   std::lock_guard<std::mutex> lk(g.mu);
   g_err.clear();
-  if (M <= 0 || !sigs48 || !neg_hms || !key_idx || !results) {
+  if (n <= 0 || !out) {
     g_err = "invalid arguments";
     return -1;
   }
+  CHK(init_locked());
+  CHK(claim_workspace_locked(g.s));
+  CHK(upload_messages_locked(msgs, offsets, n, dst, dst_len));
+  CHK(g.nhm_ref.ensure((size_t)n * 96));
+  CHK(g.status.ensure((size_t)n * 4));
+  CHK(bls::launch_hash_to_g1(g.msg_bytes.as<uint8_t>(), g.msg_offs.as<int64_t>(), n, g.dst_bytes.as<uint8_t>(),
+                             dst_len, g.nhm_ref.as<uint64_t>(), g.status.as<int32_t>(), negate, g.s));
+  HIPCHK(hipMemcpyAsync(out, g.nhm_ref.p, (size_t)n * 96, hipMemcpyDeviceToHost, g.s));
+  if (is_identity) HIPCHK(hipMemcpyAsync(is_identity, g.status.p, (size_t)n * 4, hipMemcpyDeviceToHost, g.s));
+  HIPCHK(hipStreamSynchronize(g.s));
+  return 0;
+}
+
+// shared by blsgpu_verify_compressed (neg_hms given) and blsgpu_verify_messages (the
+// messages hashed on the device, on the second stream while the signatures decode)
+static int verify_compressed_locked(const uint8_t* sigs48, const uint64_t* neg_hms, const uint8_t* msgs,
+                                    const int64_t* offsets, const uint8_t* dst, int dst_len,
+                                    const int32_t* key_idx, int32_t* results, uint64_t* gt_out, int M) {
   CHK(init_locked());
   CHK(claim_workspace_locked(g.s));
   for (int i = 0; i < M; i++)
@@ -374,9 +420,18 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
   HIPCHK(hipMemcpyAsync(g.comp.p, sigs48, m * 48, hipMemcpyHostToDevice, g.s));
-  HIPCHK(hipMemcpyAsync(g.nhm_ref.p, neg_hms, m * 96, hipMemcpyHostToDevice, g.s));
+  if (neg_hms) HIPCHK(hipMemcpyAsync(g.nhm_ref.p, neg_hms, m * 96, hipMemcpyHostToDevice, g.s));
+  else CHK(upload_messages_locked(msgs, offsets, M, dst, dst_len));
   HIPCHK(hipMemcpyAsync(g.keyidx.p, key_idx, m * 4, hipMemcpyHostToDevice, g.s));
   bls::EngineArgs a{};
+  if (!neg_hms) {  // -H(m) on the second stream, beside the signature decoding
+    HIPCHK(hipEventRecord(g.ev_fork, g.s));
+    HIPCHK(hipStreamWaitEvent(g.s2, g.ev_fork, 0));
+    CHK(bls::launch_hash_to_g1(g.msg_bytes.as<uint8_t>(), g.msg_offs.as<int64_t>(), M, g.dst_bytes.as<uint8_t>(),
+                               dst_len, g.nhm_ref.as<uint64_t>(), nullptr, 1, g.s2));
+    HIPCHK(hipEventRecord(g.ev_h2c, g.s2));
+    a.nhm_ready = g.ev_h2c;
+  }
   a.M = M;
   a.mode = g.mode;
   a.g_ml = g.g_ml;
@@ -402,6 +457,28 @@ int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, con
   if (gt_out) HIPCHK(hipMemcpyAsync(gt_out, g.gt.p, m * 72 * 8, hipMemcpyDeviceToHost, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
   return 0;
+}
+
+int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, const int32_t* key_idx,
+                             int32_t* results, uint64_t* gt_out, int M) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (M <= 0 || !sigs48 || !neg_hms || !key_idx || !results) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  return verify_compressed_locked(sigs48, neg_hms, nullptr, nullptr, nullptr, 0, key_idx, results, gt_out, M);
+}
+
+int blsgpu_verify_messages(const uint8_t* sigs48, const uint8_t* msgs, const int64_t* offsets, const uint8_t* dst,
+                           int dst_len, const int32_t* key_idx, int32_t* results, uint64_t* gt_out, int M) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (M <= 0 || !sigs48 || !key_idx || !results) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  return verify_compressed_locked(sigs48, nullptr, msgs, offsets, dst, dst_len, key_idx, results, gt_out, M);
 }
 
 int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, const int32_t* d_key_idx,

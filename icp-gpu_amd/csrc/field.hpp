@@ -313,13 +313,15 @@ __device__ __forceinline__ Fp pow_fixed(const Fp& a, const u32 (&e)[12]) {
 }
 // Fermat inverse a^(p-2) (the reference's fp_inv, gpu-crypto/cuda/field_sppark.cuh:40-50)
 __device__ __forceinline__ Fp inv(const Fp& a) { return pow_fixed(a, INV_EXP); }
-// a^((p+1)/4); ok = (result^2 == a) (ic-integration/gpu_crypto/cuda/decompress.cu:16-40)
-// Same exponent by the 4-bit sliding window of SQRT_WIN (379 squarings and 79
+// a^e for an exponent given as a 4-bit sliding window (a table of gen_constants.py);
+// sqrt_cand: a^((p+1)/4) (ic-integration/gpu_crypto/cuda/decompress.cu:16-40), the
+// same exponent by the 4-bit sliding window of SQRT_WIN (379 squarings and 79
 // multiplications instead of 384 and 229): odd powers a, a^3, .., a^15 first, then one
 // multiplication per window.  The window walk is the same on every lane (the exponent is
 // a constant), so the digit selection is a scalar branch around register copies and
 // the loop holds a single multiplier body.
-__device__ __forceinline__ Fp sqrt_cand(const Fp& a) {
+template <int LEN>
+__device__ __forceinline__ Fp pow_win(const Fp& a, const uint8_t (&win)[LEN][2]) {
   Fp pw[8];
   pw[0] = a;
   const Fp a2 = sqr(a);
@@ -333,16 +335,17 @@ __device__ __forceinline__ Fp sqrt_cand(const Fp& a) {
       if ((d >> 1) == k) t = pw[k];
     return t;
   };
-  Fp r = pick(SQRT_WIN[0][1]);
+  Fp r = pick(win[0][1]);
 #pragma unroll 1
-  for (int w = 1; w < SQRT_WIN_LEN; w++) {
-    const int nsq = SQRT_WIN[w][0], d = SQRT_WIN[w][1];
+  for (int w = 1; w < LEN; w++) {
+    const int nsq = win[w][0], d = win[w][1];
 #pragma unroll 1
     for (int q = 0; q < nsq; q++) r = sqr(r);
     if (d) r = mul(r, pick(d));
   }
   return r;
 }
+__device__ __forceinline__ Fp sqrt_cand(const Fp& a) { return pow_win(a, SQRT_WIN); }
 
 // ---- inverse by batched divsteps (Bernstein-Yang "safegcd", eprint 2019/266) ----
 // Variable time (every input here is public).  Works on the canonical value

@@ -48,6 +48,142 @@ def f2pow(a, e):
     return r
 
 
+# ---- hash to G1 (RFC 9380, suite BLS12381G1_XMD:SHA-256_SSWU_RO_) ----
+# The simplified SWU map runs on E': y^2 = x^3 + A'x + B' (RFC 9380 sec. 8.8.1), whose
+# image reaches E: y^2 = x^3 + 4 through an isogeny of degree 11.  The isogeny's
+# rational maps are derived here by Velu's formulas from the 11-torsion of E', not
+# copied: E'(Fp) has the order of E(Fp) (checked) and a unique subgroup of order 11;
+# the normalised isogeny with that kernel lands on y^2 = x^3 + 4 * 11^6, and
+# (x, y) -> (x / 11^2, y / 11^3) maps that onto E.  Two coefficients the RFC publishes
+# (appendix E.2: k_(1,0) and k_(3,0)) are checked against the derived maps.
+SSWU_A = 0x144698a3b8e9433d693a02c96d4982b0ea985383ee66a8d8e8981aefd881ac98936f8da0e0f97f5cf428082d584c1d
+SSWU_B = 0x12e2908d11688030018b12e8753eee3b2016c1f0f24f4070a0b9c14fcef35ef55a23215a316ceaa5d1cc48e98e172be0
+SSWU_Z = 11
+RFC_K10 = 0x11a05f2b1e833340b809101dd99815856b303e88a2d7005ff2627b56cdb4e2c85610c2d5f2e62d6eaeac1662734649b7
+RFC_K30 = 0x90d97c81ba24ee0259d1f094980dcfa11ad138e48a869522b52af6c956543d3cd0c7aee9b3ba3c2be9845719707bb33
+
+
+def _ec_add(p1, p2, a):  # affine points on y^2 = x^3 + a x + b; None = identity
+    if p1 is None:
+        return p2
+    if p2 is None:
+        return p1
+    (x1, y1), (x2, y2) = p1, p2
+    if x1 == x2:
+        if (y1 + y2) % P == 0:
+            return None
+        lam = (3 * x1 * x1 + a) * pow(2 * y1, -1, P) % P
+    else:
+        lam = (y2 - y1) * pow(x2 - x1, -1, P) % P
+    x3 = (lam * lam - x1 - x2) % P
+    return (x3, (lam * (x1 - x3) - y1) % P)
+
+
+def _ec_mul(pt, k, a):
+    acc = None
+    for bit in bin(k)[2:]:
+        acc = _ec_add(acc, acc, a)
+        if bit == "1":
+            acc = _ec_add(acc, pt, a)
+    return acc
+
+
+def _poly_mul(f, g):  # coefficient lists, lowest degree first, mod P
+    o = [0] * (len(f) + len(g) - 1)
+    for i, x in enumerate(f):
+        for j, y in enumerate(g):
+            o[i + j] = (o[i + j] + x * y) % P
+    return o
+
+
+def _poly_add(f, g, sign=1):
+    n = max(len(f), len(g))
+    return [((f[i] if i < len(f) else 0) + sign * (g[i] if i < len(g) else 0)) % P for i in range(n)]
+
+
+def _poly_der(f):
+    return [i * f[i] % P for i in range(1, len(f))]
+
+
+def _poly_mod(f, g):  # g monic
+    f = f[:]
+    while len(f) >= len(g):
+        c, d = f[-1], len(f) - len(g)
+        for i, y in enumerate(g):
+            f[i + d] = (f[i + d] - c * y) % P
+        f.pop()
+    return f
+
+
+def iso11_maps():
+    """(x_num, x_den, y_num, y_den) of the 11-isogeny E' -> E, lowest degree first;
+    x_den and y_den are monic of degrees 10 and 15."""
+    a, b = SSWU_A, SSWU_B
+    n = P + 1 - (1 - BLS_X)  # #E(Fp), trace t = x + 1 = 1 - BLS_X
+    x0 = 1
+    while True:  # a point of E' (deterministic), then its multiple of order 11
+        rhs = (x0 ** 3 + a * x0 + b) % P
+        y0 = pow(rhs, (P + 1) // 4, P)
+        if y0 * y0 % P == rhs:
+            pt = (x0, y0)
+            assert _ec_mul(pt, n, a) is None  # #E'(Fp) = #E(Fp)
+            q = _ec_mul(pt, n // 11, a)
+            if q is not None:
+                break
+        x0 += 1
+    assert _ec_mul(q, 11, a) is None
+    kern = [_ec_mul(q, i, a) for i in range(1, 6)]  # one point of each +- pair
+    dpoly = [1]
+    for xq, _ in kern:
+        dpoly = _poly_mul(dpoly, [(-xq) % P, 1])
+    # Velu: with v_Q = 2(3 x_Q^2 + a), u_Q = 4 y_Q^2 over the five pairs,
+    # X = x + sum v_Q / (x - x_Q) + u_Q / (x - x_Q)^2, codomain A = a - 5 sum v_Q,
+    # B = b - 7 sum (u_Q + x_Q v_Q).  sum f(x_Q) / (x - x_Q) = ((f D') mod D) / D.
+    v = sum(2 * (3 * xq * xq + a) for xq, _ in kern) % P
+    w = sum(4 * (xq ** 3 + a * xq + b) + xq * 2 * (3 * xq * xq + a) for xq, _ in kern) % P
+    assert (a - 5 * v) % P == 0 and (b - 7 * w) % P == 4 * 11 ** 6
+    dd = _poly_der(dpoly)
+    s1 = _poly_mod(_poly_mul([2 * a % P, 0, 6], dd), dpoly)
+    s2 = _poly_mod(_poly_mul([4 * b % P, 4 * a % P, 0, 4], dd), dpoly)
+    d2 = _poly_mul(dpoly, dpoly)
+    # X = N / D^2 with N = x D^2 + s1 D - (s2' D - s2 D')
+    num = _poly_add(_poly_add(_poly_mul([0, 1], d2), _poly_mul(s1, dpoly)),
+                    _poly_add(_poly_mul(_poly_der(s2), dpoly), _poly_mul(s2, dd), -1), -1)
+    # normalised: Y = y X'(x) = y (N' D - 2 N D') / D^3
+    ynum = _poly_add(_poly_mul(_poly_der(num), dpoly), _poly_mul(num, dd), -2)
+    i2, i3 = pow(121, -1, P), pow(1331, -1, P)
+    xn, yn = [c * i2 % P for c in num], [c * i3 % P for c in ynum]
+    xd, yd = d2, _poly_mul(d2, dpoly)
+    assert len(xn) == 12 and len(xd) == 11 and len(yn) == 16 and len(yd) == 16 and xd[-1] == yd[-1] == 1
+    assert xn[0] == RFC_K10 and yn[0] == RFC_K30
+    return xn, xd, yn, yd
+
+
+def window4(e):
+    """e as a left-to-right sliding window of 4 bits: the first entry's odd digit
+    starts the result; every later entry squares `sq` times, then multiplies by
+    a^digit (digit odd, < 16); a final entry with digit 0 only squares"""
+    bits = bin(e)[2:]
+    win, i, pend = [], 0, 0
+    while i < len(bits):
+        if bits[i] == "0":
+            pend += 1
+            i += 1
+            continue
+        j = min(i + 4, len(bits))
+        while bits[j - 1] == "0":
+            j -= 1
+        win.append((pend + (j - i), int(bits[i:j], 2)))
+        pend, i = 0, j
+    if pend:
+        win.append((pend, 0))
+    chk = 0
+    for k, (sq, d) in enumerate(win):
+        chk = d if k == 0 else (chk << sq) + d
+    assert chk == e and all(d % 2 == 1 for _, d in win if d) and win[0][1] > 0
+    return win
+
+
 def main():
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
 
@@ -153,31 +289,27 @@ def main():
     emit("constexpr uint32_t P32[12] = {%s};" % words(P, 12))
     emit("constexpr uint32_t R_ORDER32[8] = {%s};" % words(R_ORDER, 8))
     emit("constexpr uint64_t BLS_X = 0x%016xull;  // x = -BLS_X" % BLS_X)
-    # (p+1)/4 as a left-to-right sliding window of 4 bits: the first entry's odd digit
-    # starts the result; every later entry squares `sq` times, then multiplies by
-    # a^digit (digit odd, < 16); a final entry with digit 0 only squares
-    e = (P + 1) // 4
-    bits = bin(e)[2:]
-    win, i, pend = [], 0, 0
-    while i < len(bits):
-        if bits[i] == "0":
-            pend += 1
-            i += 1
-            continue
-        j = min(i + 4, len(bits))
-        while bits[j - 1] == "0":
-            j -= 1
-        win.append((pend + (j - i), int(bits[i:j], 2)))
-        pend, i = 0, j
-    if pend:
-        win.append((pend, 0))
-    chk = 0
-    for k, (sq, d) in enumerate(win):
-        chk = d if k == 0 else (chk << sq) + d
-    assert chk == e and all(d % 2 == 1 for _, d in win if d) and win[0][1] > 0
-    emit("// (p+1)/4 as a 4-bit sliding window: {squarings, odd digit} (see gen_constants.py)")
-    emit("constexpr int SQRT_WIN_LEN = %d;" % len(win))
-    emit("constexpr uint8_t SQRT_WIN[%d][2] = {%s};" % (len(win), ", ".join("{%d, %d}" % w for w in win)))
+    # exponents as left-to-right sliding windows of 4 bits
+    for name, e, what in (("SQRT_WIN", (P + 1) // 4, "(p+1)/4"), ("P34_WIN", (P - 3) // 4, "(p-3)/4")):
+        win = window4(e)
+        emit("// %s as a 4-bit sliding window: {squarings, odd digit} (see gen_constants.py)" % what)
+        emit("constexpr int %s_LEN = %d;" % (name, len(win)))
+        emit("constexpr uint8_t %s[%d][2] = {%s};" % (name, len(win), ", ".join("{%d, %d}" % w for w in win)))
+    # hash to G1: SSWU on E' and the 11-isogeny to E
+    arr("SSWU_A_M", to_int(SSWU_A), "A' of E' (RFC 9380 sec. 8.8.1)")
+    arr("SSWU_B_M", to_int(SSWU_B), "B' of E'")
+    arr("SSWU_Z_M", to_int(SSWU_Z), "Z = 11")
+    c2 = pow((-SSWU_Z) % P, (P + 1) // 4, P)
+    assert c2 * c2 % P == (-SSWU_Z) % P
+    arr("SSWU_SQRT_MZ_M", to_int(c2), "a square root of -Z")
+    arr("C_H2F_HI", pow(2, 384, P) * RI * RI % P, "2^384 R'^2 mod p: mont(hi, this) = hi 2^384 R' for the top of a 512-bit value")
+    maps = iso11_maps()
+    for name, poly in zip(("ISO_XNUM", "ISO_XDEN", "ISO_YNUM", "ISO_YDEN"), maps):
+        coeffs = poly if name.endswith("NUM") else poly[:-1]  # denominators are monic
+        emit("// %s of the 11-isogeny E' -> E, lowest degree first%s" %
+             (name[4:].lower(), "" if name.endswith("NUM") else " (monic: the leading 1 is implied)"))
+        emit("constexpr int32_t %s[%d][14] = {%s};" % (name, len(coeffs), ", ".join(
+            "{%s}" % ", ".join("0x%07x" % x for x in limbs28(to_int(c))) for c in coeffs)))
     emit("}  // namespace bls")
     with open(os.path.join(HERE, "field_constants.hpp"), "w") as f:
         f.write("\n".join(out) + "\n")

@@ -14,6 +14,11 @@ int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t 
 int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, int soa, hipStream_t s);
 int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s);
 int launch_g2_prepare(const uint64_t* q, int32_t* out, int K, hipStream_t s);
+// n messages (bytes msgs[offs[i] .. offs[i+1])) -> H(m) (RFC 9380 hash to G1 with the
+// given DST), or -H(m) when negate != 0, as n x 12 u64 affine points in reference form;
+// status[i] (nullable) = 1 for the identity
+int launch_hash_to_g1(const uint8_t* msgs, const int64_t* offs, int n, const uint8_t* dst, int dlen,
+                      uint64_t* out, int32_t* status, int negate, hipStream_t s);
 // K compressed G2 points (96 bytes each) -> reference-form affine points + decode status
 int launch_g2_decompress(const uint8_t* in, uint64_t* q_ref, int32_t* status, int K, int check_subgroup,
                          hipStream_t s);

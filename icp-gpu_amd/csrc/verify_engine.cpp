@@ -66,6 +66,7 @@ int engine_compressed(const EngineArgs& a, hipStream_t s) {
   if (mark(a, 0, s)) return -1;
   if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 2, s)) return -1;
   if (launch_g1_decode(a.comp, a.sig_pts, a.status, a.M, s)) return -1;
+  if (a.nhm_ready && hipStreamWaitEvent(s, a.nhm_ready, 0) != hipSuccess) return -1;
   if (mark(a, 1, s) || launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
   const bool tiles_ml = !(a.queue && a.wpc_ml >= 0), tiles_fe = !(a.queue && a.wpc_fe >= 0);
   if (a.mode == 1) {

@@ -45,6 +45,9 @@ struct EngineArgs {
   // second stream for the subgroup test, and the fork/join events (fused path)
   hipStream_t s2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // set when -H(m) is produced on another stream (hashed on the device): the engine
+  // waits for it before reading nhm_ref
+  hipEvent_t nhm_ready = nullptr;
 };
 
 int engine_legacy(const EngineArgs& a, hipStream_t s);

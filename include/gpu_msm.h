@@ -103,6 +103,22 @@ void blsgpu_reset_keys(void);
 int blsgpu_verify_compressed(const uint8_t* sigs48, const uint64_t* neg_hms, const int32_t* key_idx,
                              int32_t* results, uint64_t* gt_out, int M);
 
+/* Hash to G1 on the GPU: RFC 9380 suite BLS12381G1_XMD:SHA-256_SSWU_RO_ with the
+ * caller's DST (1..255 bytes; the reference's benchmark uses
+ * "BLS_SIG_BLS12381G2_XMD:SHA-256_SSWU_RO_NUL_", ic-bench/src/main.rs:68), the
+ * computation ic_bls12_381's hash_to_curve performs on the CPU before the reference's
+ * GPU step (patch:454-457).  Message i is msgs[offsets[i] .. offsets[i+1]) (offsets: n+1
+ * entries).  out: n x 12 u64, H(m) affine in the G1 boundary form, or -H(m) when
+ * negate != 0; is_identity (nullable) flags the (never observed) identity, output zeros. */
+int blsgpu_hash_to_g1(const uint8_t* msgs, const int64_t* offsets, int n, const uint8_t* dst, int dst_len,
+                      uint64_t* out, int32_t* is_identity, int negate);
+
+/* Fused wire-format path: compressed signatures, messages and key handles in; H(m) is
+ * computed on the device (as blsgpu_hash_to_g1, beside the signature decoding) and the
+ * rest is blsgpu_verify_compressed (same results and gt_out). */
+int blsgpu_verify_messages(const uint8_t* sigs48, const uint8_t* msgs, const int64_t* offsets, const uint8_t* dst,
+                           int dst_len, const int32_t* key_idx, int32_t* results, uint64_t* gt_out, int M);
+
 /* Decompress with per-point status (n x 96 bytes out; status[i] as BLSGPU_ERR_*, 0 ok,
  * 1 identity).  check_subgroup = 0 skips the subgroup test. */
 int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* status, int n,
