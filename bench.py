@@ -16,6 +16,10 @@ Beside that headline step the same run measures the other forms on their workloa
             (one graph-replayed sequence with queue-fed persistent kernels, autotuned);
   at_scale  65536 verifications over 40 keys and 4096 messages, lognormal batch sizes
             (median 64, sigma 1.2, [1, 8192]), batched form (and one plain pass);
+  wire      the fused wire-format path on a uniform 4096 batch: compressed signatures,
+            32-byte messages and key handles in, H(m) hashed on the device (RFC 9380 with
+            the reference benchmark's DST), against the compressed path fed -H(m) from
+            the host; host entry points, copies included;
   serving   Poisson arrivals at 2k, 20k and 100k verifications/s, lognormal request
             sizes (median 16, sigma 1.0, [1, 512]) over 40 keys, 1/256 invalid, under a
             4 GB HBM budget: p50/p99/p999 latency and sustained throughput of the
@@ -58,7 +62,7 @@ def parse():
                     help="mode:lanes_miller:lanes_final_exp, or auto (the library's autotuner on this batch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
-    ap.add_argument("--forms", default="plain,batched,at_scale,serving",
+    ap.add_argument("--forms", default="plain,wire,batched,at_scale,serving",
                     help="comma list of forms to measure beside the headline plain form")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of each serving run (60 for the full runs)")
@@ -287,6 +291,68 @@ def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune):
     }
 
 
+IC_DST = b"BLS_SIG_BLS12381G2_XMD:SHA-256_SSWU_RO_NUL_"  # ic-bench/src/main.rs:68
+
+
+def measure_wire(lib, wl, blsgpu, sk, handles, key_idx, seed, steps):
+    vp = ctypes.c_void_p
+    p = lambda a: a.ctypes.data_as(vp)
+    n = len(key_idx)
+    msgs = []
+    buf = np.zeros(32, np.uint8)
+    for i in range(n):
+        wl.blsgpu_wl_message(ctypes.c_uint64(seed), ctypes.c_int64(i), p(buf))
+        msgs.append(buf.tobytes())
+    h, _ = blsgpu.hash_to_g1(msgs, IC_DST)
+    sig = np.zeros((n, 48), np.uint8)
+    kidx = np.ascontiguousarray(key_idx, np.int32)
+    wl.blsgpu_wl_sign_points.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+    if wl.blsgpu_wl_sign_points(p(h), n, p(sk), sk.shape[0], p(kidx), p(sig)):
+        raise RuntimeError("sign: " + wl.blsgpu_wl_last_error().decode())
+    want = np.ones(n, np.int32)
+    for i in range(0, n, 64):  # 1/64 of the messages changed after signing
+        msgs[i] = bytes([msgs[i][0] ^ 1]) + msgs[i][1:]
+        want[i] = 0
+    offs = np.zeros(n + 1, np.int64)
+    offs[1:] = np.cumsum([len(m) for m in msgs])
+    mb = np.frombuffer(b"".join(msgs), np.uint8).copy()
+    dst = np.frombuffer(IC_DST, np.uint8).copy()
+    hk = np.ascontiguousarray(handles[kidx], np.int32)
+    res = np.zeros(n, np.int32)
+    nh = np.zeros((n, 12), np.uint64)
+    ident = np.zeros(n, np.int32)
+
+    def wire():
+        if lib.blsgpu_verify_messages(p(sig), p(mb), p(offs), p(dst), len(IC_DST), p(hk), p(res), None, n):
+            raise RuntimeError(blsgpu.last_error())
+
+    def hash_only():
+        if lib.blsgpu_hash_to_g1(p(mb), p(offs), n, p(dst), len(IC_DST), p(nh), p(ident), 1):
+            raise RuntimeError(blsgpu.last_error())
+
+    def compressed():
+        if lib.blsgpu_verify_compressed(p(sig), p(nh), p(hk), p(res), None, n):
+            raise RuntimeError(blsgpu.last_error())
+
+    def timed(fn):
+        fn()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    ms_wire = timed(wire)
+    ok = bool((res == want).all())
+    ms_hash = timed(hash_only)
+    ms_comp = timed(compressed)
+    ok = ok and bool((res == want).all())
+    return {"verif_per_s": n / (ms_wire * 1e-3), "ms_per_step": ms_wire, "hash_ms": ms_hash,
+            "compressed_ms": ms_comp, "steps": steps, "verdicts_as_expected": ok,
+            "workload": "%d verifications, %d keys, distinct 32-byte messages, 1/64 signed over another message; "
+                        "host entry points (copies included): blsgpu_verify_messages, blsgpu_hash_to_g1 alone, "
+                        "blsgpu_verify_compressed fed the same -H(m)" % (n, int(sk.shape[0]))}
+
+
 def measure_serving(blsgpu, pool, handles, rates, seconds, seed):
     """continuous-batching server vs the plain one-request-per-step form, per arrival rate"""
     keys = handles[pool["key_idx"]].astype(np.int32)
@@ -468,6 +534,10 @@ def main():
     }
     forms = set(f for f in args.forms.split(",") if f)
     t_forms = time.perf_counter()
+    if "wire" in forms:
+        log("wire-format path")
+        out["wire"] = measure_wire(lib, wl, blsgpu, sk, handles, key_idx, args.seed + 6, 10)
+        log("wire: %s" % {k: out["wire"][k] for k in ("verif_per_s", "hash_ms", "compressed_ms")})
     if "batched" in forms or "at_scale" in forms or "serving" in forms:
         blsgpu.set_batched_engine(12, 16, 0, 0, True)
     if "batched" in forms:

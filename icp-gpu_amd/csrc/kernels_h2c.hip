@@ -15,29 +15,46 @@ __device__ __forceinline__ void st_ref_h2c(u64* p, const Fp& a) {
 
 // message i = msgs[offs[i] .. offs[i+1]) -> H(m) (or -H(m) when negate != 0) as an
 // affine point in reference form (12 u64: x then y); status[i] = 0, or 1 and zeros for
-// the identity (nullable)
-__global__ void k_hash_to_g1(const uint8_t* __restrict__ msgs, const int64_t* __restrict__ offs, int n,
-                             const uint8_t* __restrict__ dst, int dlen, u64* __restrict__ out,
-                             i32* __restrict__ status, int negate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t o = offs[i];
+// the identity (nullable).  Two lanes per message: both expand the message, lane 0 maps
+// u0 and lane 1 u1 onto the curve at the same time; lane 1 hands its point over through
+// LDS and lane 0 adds, clears the cofactor and converts to affine.
+constexpr int H2C_BLOCK = 64;
+__global__ __launch_bounds__(H2C_BLOCK) void k_hash_to_g1(const uint8_t* __restrict__ msgs,
+                                                         const int64_t* __restrict__ offs, int n,
+                                                         const uint8_t* __restrict__ dst, int dlen,
+                                                         u64* __restrict__ out, i32* __restrict__ status,
+                                                         int negate) {
+  __shared__ Jac<Fp> q1s[H2C_BLOCK / 2];
+  const int t = blockIdx.x * H2C_BLOCK + threadIdx.x;
+  const int i = t >> 1, lane = t & 1, slot = threadIdx.x >> 1;
+  const bool active = i < n;
+  Jac<Fp> q;
+  if (active) {
+    const int64_t o = offs[i];
+    Fp u0, u1;
+    hash_to_field2(msgs + o, (int)(offs[i + 1] - o), dst, dlen, u0, u1);
+    q = map_to_g1_curve<FieldInline>(lane ? u1 : u0);
+    if (lane) q1s[slot] = q;
+  }
+  __syncthreads();
+  if (!active || lane) return;
   Fp x, y;
-  const bool ok = hash_to_g1(msgs + o, (int)(offs[i + 1] - o), dst, dlen, x, y);
-  u64* q = out + (size_t)i * 12;
+  const bool ok = clear_cofactor_affine(q, q1s[slot], x, y);
+  u64* r = out + (size_t)i * 12;
   if (ok) {
-    st_ref_h2c(q, x);
-    st_ref_h2c(q + 6, negate ? neg(y) : y);
+    st_ref_h2c(r, x);
+    st_ref_h2c(r + 6, negate ? neg(y) : y);
   } else {
-    for (int j = 0; j < 12; j++) q[j] = 0;
+    for (int j = 0; j < 12; j++) r[j] = 0;
   }
   if (status) status[i] = ok ? 0 : 1;
 }
 
 int launch_hash_to_g1(const uint8_t* msgs, const int64_t* offs, int n, const uint8_t* dst, int dlen,
                       uint64_t* out, int32_t* status, int negate, hipStream_t s) {
-  hipLaunchKernelGGL(k_hash_to_g1, dim3((n + 63) / 64), dim3(64), 0, s, msgs, offs, n, dst, dlen, out, status,
-                     negate);
+  const long long threads = 2LL * n;
+  hipLaunchKernelGGL(k_hash_to_g1, dim3((unsigned)((threads + H2C_BLOCK - 1) / H2C_BLOCK)), dim3(H2C_BLOCK), 0, s, msgs,
+                     offs, n, dst, dlen, out, status, negate);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -200,6 +200,29 @@ __global__ void k_wl_items(const uint8_t* __restrict__ hs, const uint8_t* __rest
   }
 }
 
+// signatures sk[key_idx[i]] * H_i of given points H_i (reference form, 12 u64 each),
+// compressed (48 bytes)
+__global__ void k_wl_sign_points(const u64* __restrict__ h_ref, const uint8_t* __restrict__ sk,
+                                 const int32_t* __restrict__ key_idx, uint8_t* __restrict__ sig48, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 w[12], kk[8];
+  const u64* h = h_ref + (size_t)i * 12;
+  Aff<Fp> H;
+  for (int j = 0; j < 6; j++) { w[2 * j] = (u32)h[j]; w[2 * j + 1] = (u32)(h[j] >> 32); }
+  H.x = from_ref(w);
+  for (int j = 0; j < 6; j++) { w[2 * j] = (u32)h[6 + j]; w[2 * j + 1] = (u32)(h[6 + j] >> 32); }
+  H.y = from_ref(w);
+  ld_scalar(sk + (size_t)key_idx[i] * 32, kk);
+  u32 be[12];
+  g1_compress(to_affine(smul(H, kk, fp_one())), false, be);
+  uint8_t* o = sig48 + (size_t)i * 48;
+  for (int j = 0; j < 12; j++) {
+    o[4 * j] = (uint8_t)(be[j] >> 24); o[4 * j + 1] = (uint8_t)(be[j] >> 16);
+    o[4 * j + 2] = (uint8_t)(be[j] >> 8); o[4 * j + 3] = (uint8_t)be[j];
+  }
+}
+
 // v_mad_u64_u32 issue-rate probe: 8 independent chains per lane, 64 per loop trip
 __global__ void k_probe_mad(u64* out, u32 seed, int iters) {
   u32 x = seed + threadIdx.x, y = seed * 3 + blockIdx.x;
@@ -264,6 +287,32 @@ int blsgpu_wl_public_keys(const uint8_t* sk32, int n, uint64_t* pk_ref) {
   hipLaunchKernelGGL(k_wl_keys, dim3((n + 63) / 64), dim3(64), 0, 0, dsk, dpk, n);
   WLCHK(hipGetLastError());
   WLCHK(hipMemcpy(pk_ref, dpk, (size_t)n * 24 * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int blsgpu_wl_sign_points(const uint64_t* h_ref, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
+                          uint8_t* sig48) {
+  std::lock_guard<std::mutex> lk(wl_mu);
+  if (n <= 0 || nkeys <= 0) return -1;
+  for (int i = 0; i < n; i++)
+    if (key_idx[i] < 0 || key_idx[i] >= nkeys) {
+      wl_err = "bad key index";
+      return -1;
+    }
+  Scoped m;
+  u64* dh;
+  uint8_t *dsk, *dsig;
+  int32_t* dki;
+  WLCHK(m.alloc(&dh, (size_t)n * 96));
+  WLCHK(m.alloc(&dsk, (size_t)nkeys * 32));
+  WLCHK(m.alloc(&dki, (size_t)n * 4));
+  WLCHK(m.alloc(&dsig, (size_t)n * 48));
+  WLCHK(hipMemcpy(dh, h_ref, (size_t)n * 96, hipMemcpyHostToDevice));
+  WLCHK(hipMemcpy(dsk, sk32, (size_t)nkeys * 32, hipMemcpyHostToDevice));
+  WLCHK(hipMemcpy(dki, key_idx, (size_t)n * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_wl_sign_points, dim3((n + 63) / 64), dim3(64), 0, 0, dh, dsk, dki, dsig, n);
+  WLCHK(hipGetLastError());
+  WLCHK(hipMemcpy(sig48, dsig, (size_t)n * 48, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -31,6 +31,10 @@ int blsgpu_wl_items(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const 
  * signers); msg_idx NULL = item index */
 int blsgpu_wl_items_msg(uint64_t seed, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
                         const int64_t* msg_idx, const int8_t* category, uint8_t* sig48, uint64_t* nhm_ref);
+/* compressed signatures sk32[key_idx[i]] * H_i of given G1 points (n x 12 u64,
+ * reference form), e.g. hashes from blsgpu_hash_to_g1 (device) */
+int blsgpu_wl_sign_points(const uint64_t* h_ref, int n, const uint8_t* sk32, int nkeys, const int32_t* key_idx,
+                          uint8_t* sig48);
 /* measured v_mad_u64_u32 issue rate over all CUs (products per second) */
 int blsgpu_probe_mad_rate(double* mads_per_s, double* ms);
 const char* blsgpu_wl_last_error(void);
