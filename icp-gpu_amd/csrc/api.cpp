@@ -39,6 +39,60 @@ const uint64_t G2_GEN_REF[24] = {
 
 }  // namespace
 
+// Small host arrays of a blocking call travel through one pinned staging area: one CPU
+// copy, then DMA from pinned memory (the runtime stages pageable copies itself, at a
+// fixed cost per copy that dominated these calls).  Every such call synchronizes before
+// it returns, so the area is free again at the next one.  Large arrays (the per-item line
+// tables of the legacy entry point) are copied directly.
+struct Staging {
+  static constexpr size_t kMax = size_t(8) << 20;
+  size_t off = 0;
+  struct Out {
+    void* host;
+    size_t at, n;
+  };
+  std::vector<Out> outs;
+  static size_t span(size_t n) { return n <= kMax ? (n + 255) & ~size_t(255) : 0; }
+  int begin(size_t total) {
+    off = 0;
+    outs.clear();
+    return g.stage.ensure(total + 4096);
+  }
+  int h2d(void* dev, const void* host, size_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    const void* src = host;
+    if (n <= kMax && off + span(n) <= g.stage.cap) {
+      uint8_t* p = g.stage.as<uint8_t>() + off;
+      std::memcpy(p, host, n);
+      off += span(n);
+      src = p;
+    }
+    if (hipMemcpyAsync(dev, src, n, hipMemcpyHostToDevice, s) != hipSuccess) {
+      g_err = "host to device copy failed";
+      return -1;
+    }
+    return 0;
+  }
+  int d2h(void* host, const void* dev, size_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    void* dstp = host;
+    if (n <= kMax && off + span(n) <= g.stage.cap) {
+      outs.push_back({host, off, n});
+      dstp = g.stage.as<uint8_t>() + off;
+      off += span(n);
+    }
+    if (hipMemcpyAsync(dstp, dev, n, hipMemcpyDeviceToHost, s) != hipSuccess) {
+      g_err = "device to host copy failed";
+      return -1;
+    }
+    return 0;
+  }
+  // after the stream has synchronized
+  void finish() {
+    for (const Out& o : outs) std::memcpy(o.host, g.stage.as<uint8_t>() + o.at, o.n);
+  }
+};
+
 // Workspace buffers are shared by all calls; before they are reused (or grown) from a
 // different stream, wait for the stream that last used them.
 int claim_workspace_locked(hipStream_t s) {
@@ -91,10 +145,13 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   CHK(g.res.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
-  HIPCHK(hipMemcpyAsync(g.sig_ref.p, sigs, m * 96, hipMemcpyHostToDevice, g.s));
-  HIPCHK(hipMemcpyAsync(g.nhm_ref.p, nhms, m * 96, hipMemcpyHostToDevice, g.s));
-  HIPCHK(hipMemcpyAsync(g.g2_ref.p, g2c, 2448 * 8, hipMemcpyHostToDevice, g.s));
-  HIPCHK(hipMemcpyAsync(g.pk_ref.p, pkc, m * 2448 * 8, hipMemcpyHostToDevice, g.s));
+  Staging st;
+  CHK(st.begin(2 * Staging::span(m * 96) + Staging::span(2448 * 8) + Staging::span(m * 2448 * 8) +
+               Staging::span(m * 4) + (gt_out ? Staging::span(m * 72 * 8) : 0)));
+  CHK(st.h2d(g.sig_ref.p, sigs, m * 96, g.s));
+  CHK(st.h2d(g.nhm_ref.p, nhms, m * 96, g.s));
+  CHK(st.h2d(g.g2_ref.p, g2c, 2448 * 8, g.s));
+  CHK(st.h2d(g.pk_ref.p, pkc, m * 2448 * 8, g.s));
   bls::EngineArgs a{};
   a.M = M;
   a.mode = g.mode;
@@ -112,9 +169,10 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   a.results = g.res.as<int32_t>();
   a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
   CHK(bls::engine_legacy(a, g.s));
-  HIPCHK(hipMemcpyAsync(results, g.res.p, m * 4, hipMemcpyDeviceToHost, g.s));
-  if (gt_out) HIPCHK(hipMemcpyAsync(gt_out, g.gt.p, m * 72 * 8, hipMemcpyDeviceToHost, g.s));
+  CHK(st.d2h(results, g.res.p, m * 4, g.s));
+  if (gt_out) CHK(st.d2h(gt_out, g.gt.p, m * 72 * 8, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
+  st.finish();
   return 0;
 }
 
@@ -177,7 +235,9 @@ int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* st
   CHK(g.status.ensure(m * 4));
   CHK(g.sub_status.ensure(m * 4));
   CHK(g.sig_ref.ensure(m * 96));
-  HIPCHK(hipMemcpyAsync(g.comp.p, in48, m * 48, hipMemcpyHostToDevice, g.s));
+  Staging st;
+  CHK(st.begin(Staging::span(m * 48) + Staging::span(m * 96) + Staging::span(m * 4)));
+  CHK(st.h2d(g.comp.p, in48, m * 48, g.s));
   {
     bls::EngineArgs a{};
     a.M = n;
@@ -188,9 +248,10 @@ int blsgpu_g1_decompress_status(const uint8_t* in48, uint8_t* out96, int32_t* st
     CHK(bls::engine_decompress(a, check_subgroup, g.s));
   }
   CHK(bls::launch_points_to_ref(g.sig_pts.as<int32_t>(), g.status.as<int32_t>(), g.sig_ref.as<uint64_t>(), n, g.s));
-  if (out96) HIPCHK(hipMemcpyAsync(out96, g.sig_ref.p, m * 96, hipMemcpyDeviceToHost, g.s));
-  if (status) HIPCHK(hipMemcpyAsync(status, g.status.p, m * 4, hipMemcpyDeviceToHost, g.s));
+  if (out96) CHK(st.d2h(out96, g.sig_ref.p, m * 96, g.s));
+  if (status) CHK(st.d2h(status, g.status.p, m * 4, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
+  st.finish();
   return 0;
 }
 
@@ -350,8 +411,8 @@ void blsgpu_reset_keys(void) {
 
 // messages (bytes msgs[offsets[i] .. offsets[i+1]) of n) and the DST, copied to the
 // device on g.s; checks the offsets and the DST length (1..255, RFC 9380 sec. 5.3.1)
-static int upload_messages_locked(const uint8_t* msgs, const int64_t* offsets, int n, const uint8_t* dst,
-                                  int dst_len) {
+static int upload_messages_locked(Staging& st, const uint8_t* msgs, const int64_t* offsets, int n,
+                                  const uint8_t* dst, int dst_len) {
   if (!offsets || !dst || dst_len < 1 || dst_len > 255 || offsets[0] < 0) {
     g_err = "invalid message arguments";
     return -1;
@@ -369,9 +430,9 @@ static int upload_messages_locked(const uint8_t* msgs, const int64_t* offsets, i
   CHK(g.msg_bytes.ensure(total > 0 ? total : 1));
   CHK(g.msg_offs.ensure((size_t)(n + 1) * 8));
   CHK(g.dst_bytes.ensure(256));
-  if (total > 0) HIPCHK(hipMemcpyAsync(g.msg_bytes.p, msgs, total, hipMemcpyHostToDevice, g.s));
-  HIPCHK(hipMemcpyAsync(g.msg_offs.p, offsets, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, g.s));
-  HIPCHK(hipMemcpyAsync(g.dst_bytes.p, dst, (size_t)dst_len, hipMemcpyHostToDevice, g.s));
+  CHK(st.h2d(g.msg_bytes.p, msgs, total, g.s));
+  CHK(st.h2d(g.msg_offs.p, offsets, (size_t)(n + 1) * 8, g.s));
+  CHK(st.h2d(g.dst_bytes.p, dst, (size_t)dst_len, g.s));
   return 0;
 }
 
@@ -385,14 +446,22 @@ int blsgpu_hash_to_g1(const uint8_t* msgs, const int64_t* offsets, int n, const 
   }
   CHK(init_locked());
   CHK(claim_workspace_locked(g.s));
-  CHK(upload_messages_locked(msgs, offsets, n, dst, dst_len));
+  if (!offsets) {
+    g_err = "invalid message arguments";
+    return -1;
+  }
+  Staging st;
+  CHK(st.begin(Staging::span(offsets[n] > 0 ? (size_t)offsets[n] : 0) + Staging::span((size_t)(n + 1) * 8) + 256 +
+               Staging::span((size_t)n * 96) + Staging::span((size_t)n * 4)));
+  CHK(upload_messages_locked(st, msgs, offsets, n, dst, dst_len));
   CHK(g.nhm_ref.ensure((size_t)n * 96));
   CHK(g.status.ensure((size_t)n * 4));
   CHK(bls::launch_hash_to_g1(g.msg_bytes.as<uint8_t>(), g.msg_offs.as<int64_t>(), n, g.dst_bytes.as<uint8_t>(),
                              dst_len, g.nhm_ref.as<uint64_t>(), g.status.as<int32_t>(), negate, g.s));
-  HIPCHK(hipMemcpyAsync(out, g.nhm_ref.p, (size_t)n * 96, hipMemcpyDeviceToHost, g.s));
-  if (is_identity) HIPCHK(hipMemcpyAsync(is_identity, g.status.p, (size_t)n * 4, hipMemcpyDeviceToHost, g.s));
+  CHK(st.d2h(out, g.nhm_ref.p, (size_t)n * 96, g.s));
+  if (is_identity) CHK(st.d2h(is_identity, g.status.p, (size_t)n * 4, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
+  st.finish();
   return 0;
 }
 
@@ -419,10 +488,19 @@ static int verify_compressed_locked(const uint8_t* sigs48, const uint64_t* neg_h
   CHK(g.res.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
-  HIPCHK(hipMemcpyAsync(g.comp.p, sigs48, m * 48, hipMemcpyHostToDevice, g.s));
-  if (neg_hms) HIPCHK(hipMemcpyAsync(g.nhm_ref.p, neg_hms, m * 96, hipMemcpyHostToDevice, g.s));
-  else CHK(upload_messages_locked(msgs, offsets, M, dst, dst_len));
-  HIPCHK(hipMemcpyAsync(g.keyidx.p, key_idx, m * 4, hipMemcpyHostToDevice, g.s));
+  if (!neg_hms && !offsets) {
+    g_err = "invalid message arguments";
+    return -1;
+  }
+  Staging st;
+  const size_t msg_total = neg_hms ? 0 : (offsets[M] > 0 ? (size_t)offsets[M] : 0);
+  CHK(st.begin(Staging::span(m * 48) + Staging::span(m * 96) + Staging::span(msg_total) +
+               Staging::span((m + 1) * 8) + 256 + 2 * Staging::span(m * 4) +
+               (gt_out ? Staging::span(m * 72 * 8) : 0)));
+  CHK(st.h2d(g.comp.p, sigs48, m * 48, g.s));
+  if (neg_hms) CHK(st.h2d(g.nhm_ref.p, neg_hms, m * 96, g.s));
+  else CHK(upload_messages_locked(st, msgs, offsets, M, dst, dst_len));
+  CHK(st.h2d(g.keyidx.p, key_idx, m * 4, g.s));
   bls::EngineArgs a{};
   if (!neg_hms) {  // -H(m) on the second stream, beside the signature decoding
     HIPCHK(hipEventRecord(g.ev_fork, g.s));
@@ -453,9 +531,10 @@ static int verify_compressed_locked(const uint8_t* sigs48, const uint64_t* neg_h
   a.results = g.res.as<int32_t>();
   a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
   CHK(bls::engine_compressed(a, g.s));
-  HIPCHK(hipMemcpyAsync(results, g.res.p, m * 4, hipMemcpyDeviceToHost, g.s));
-  if (gt_out) HIPCHK(hipMemcpyAsync(gt_out, g.gt.p, m * 72 * 8, hipMemcpyDeviceToHost, g.s));
+  CHK(st.d2h(results, g.res.p, m * 4, g.s));
+  if (gt_out) CHK(st.d2h(gt_out, g.gt.p, m * 72 * 8, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
+  st.finish();
   return 0;
 }
 

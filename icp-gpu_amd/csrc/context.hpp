@@ -98,6 +98,7 @@ struct Ctx {
   // per-call staging
   DevBuf sig_ref, nhm_ref, g2_ref, pk_ref, comp, keyidx;
   DevBuf msg_bytes, msg_offs, dst_bytes;  // messages and DST of the hashing entry points
+  PinnedBuf stage;                         // host staging of the blocking entry points
   DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, sub_status, fbuf, res, gt;
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_h2c = nullptr;
