@@ -409,10 +409,10 @@ void blsgpu_reset_keys(void) {
   g.nkeys = 0;
 }
 
-// messages (bytes msgs[offsets[i] .. offsets[i+1]) of n) and the DST, copied to the
-// device on g.s; checks the offsets and the DST length (1..255, RFC 9380 sec. 5.3.1)
-static int upload_messages_locked(Staging& st, const uint8_t* msgs, const int64_t* offsets, int n,
-                                  const uint8_t* dst, int dst_len) {
+// messages (bytes msgs[offsets[i] .. offsets[i+1]) of n) and the DST: the offsets and
+// the DST length (1..255, RFC 9380 sec. 5.3.1) checked; returns the bytes to copy, or -1
+static int64_t check_messages(const uint8_t* msgs, const int64_t* offsets, int n, const uint8_t* dst,
+                              int dst_len) {
   if (!offsets || !dst || dst_len < 1 || dst_len > 255 || offsets[0] < 0) {
     g_err = "invalid message arguments";
     return -1;
@@ -422,11 +422,16 @@ static int upload_messages_locked(Staging& st, const uint8_t* msgs, const int64_
       g_err = "message offsets must be non-decreasing, messages at most 16 MiB";
       return -1;
     }
-  const size_t total = (size_t)offsets[n];
-  if (total > 0 && !msgs) {
+  if (offsets[n] > 0 && !msgs) {
     g_err = "invalid message arguments";
     return -1;
   }
+  return offsets[n];
+}
+// the checked messages and DST, copied to the device on g.s
+static int upload_messages_locked(Staging& st, const uint8_t* msgs, const int64_t* offsets, int n,
+                                  const uint8_t* dst, int dst_len) {
+  const size_t total = (size_t)offsets[n];
   CHK(g.msg_bytes.ensure(total > 0 ? total : 1));
   CHK(g.msg_offs.ensure((size_t)(n + 1) * 8));
   CHK(g.dst_bytes.ensure(256));
@@ -446,12 +451,10 @@ int blsgpu_hash_to_g1(const uint8_t* msgs, const int64_t* offsets, int n, const 
   }
   CHK(init_locked());
   CHK(claim_workspace_locked(g.s));
-  if (!offsets) {
-    g_err = "invalid message arguments";
-    return -1;
-  }
+  const int64_t msg_total = check_messages(msgs, offsets, n, dst, dst_len);
+  if (msg_total < 0) return -1;
   Staging st;
-  CHK(st.begin(Staging::span(offsets[n] > 0 ? (size_t)offsets[n] : 0) + Staging::span((size_t)(n + 1) * 8) + 256 +
+  CHK(st.begin(Staging::span((size_t)msg_total) + Staging::span((size_t)(n + 1) * 8) + 256 +
                Staging::span((size_t)n * 96) + Staging::span((size_t)n * 4)));
   CHK(upload_messages_locked(st, msgs, offsets, n, dst, dst_len));
   CHK(g.nhm_ref.ensure((size_t)n * 96));
@@ -488,13 +491,10 @@ static int verify_compressed_locked(const uint8_t* sigs48, const uint64_t* neg_h
   CHK(g.res.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
-  if (!neg_hms && !offsets) {
-    g_err = "invalid message arguments";
-    return -1;
-  }
+  const int64_t msg_total = neg_hms ? 0 : check_messages(msgs, offsets, M, dst, dst_len);
+  if (msg_total < 0) return -1;
   Staging st;
-  const size_t msg_total = neg_hms ? 0 : (offsets[M] > 0 ? (size_t)offsets[M] : 0);
-  CHK(st.begin(Staging::span(m * 48) + Staging::span(m * 96) + Staging::span(msg_total) +
+  CHK(st.begin(Staging::span(m * 48) + Staging::span(m * 96) + Staging::span((size_t)msg_total) +
                Staging::span((m + 1) * 8) + 256 + 2 * Staging::span(m * 4) +
                (gt_out ? Staging::span(m * 72 * 8) : 0)));
   CHK(st.h2d(g.comp.p, sigs48, m * 48, g.s));

@@ -289,6 +289,23 @@ __device__ __forceinline__ Fp from_ref(const u32 w[12]) { return mul(unpack384(w
 __device__ __forceinline__ void to_ref(const Fp& a, u32 w[12]) {
   pack384(canon(mul(a, fp_const(C_TO_REF))), w);
 }
+// one Fp in the boundary form: 6 little-endian u64 words, Montgomery R = 2^384
+__device__ __forceinline__ Fp ld_ref(const u64* p) {
+  u32 w[12];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    const u64 v = p[j];
+    w[2 * j] = (u32)v;
+    w[2 * j + 1] = (u32)(v >> 32);
+  }
+  return from_ref(w);
+}
+__device__ __forceinline__ void st_ref(u64* p, const Fp& a) {
+  u32 w[12];
+  to_ref(a, w);
+#pragma unroll
+  for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
+}
 // plain integer (e.g. decoded from big-endian bytes) -> internal Montgomery form
 __device__ __forceinline__ Fp from_plain(const Fp& v) { return mul(v, fp_const(C_R2_INT)); }
 // internal -> canonical plain integer

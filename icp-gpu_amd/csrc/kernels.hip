@@ -41,22 +41,6 @@ __device__ __forceinline__ void st_fp_aos(i32* p, const Fp& v) {
 #pragma unroll
   for (int j = 0; j < NL; j++) p[j] = v.l[j];
 }
-__device__ __forceinline__ Fp ld_ref(const u64* p) {
-  u32 w[12];
-#pragma unroll
-  for (int j = 0; j < 6; j++) {
-    u64 v = p[j];
-    w[2 * j] = (u32)v;
-    w[2 * j + 1] = (u32)(v >> 32);
-  }
-  return from_ref(w);
-}
-__device__ __forceinline__ void st_ref(u64* p, const Fp& a) {
-  u32 w[12];
-  to_ref(a, w);
-#pragma unroll
-  for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
-}
 
 // ---- line sources for the Miller loop ----
 struct LinesAoS {  // one key table (same for every lane or chosen per lane)

@@ -60,13 +60,6 @@ __device__ __forceinline__ void lds_st(int off, const Fp& v) {
   for (int j = 0; j < 7; j++) q[j] = make_int2(v.l[2 * j], v.l[2 * j + 1]);
 }
 
-__device__ __forceinline__ void st_ref_words(u64* p, const Fp& a) {
-  u32 w[12];
-  to_ref(a, w);
-#pragma unroll
-  for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
-}
-
 // word offsets of the four operand regions of a schedule: A, B, T (products), K (constants)
 struct Regions {
   int a, b, t, k;
@@ -766,7 +759,7 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
         const Fp v = canon(lds_ld(R0 + c * SLOT));
         const Fp want = c == 0 ? canon(fp_one()) : fp_zero();
         for (int l = 0; l < NL; l++) ok &= v.l[l] == want.l[l];
-        if (gt_out) st_ref_words(gt_out + (size_t)it * 72 + c * 6, lds_ld(R0 + c * SLOT));
+        if (gt_out) st_ref(gt_out + (size_t)it * 72 + c * 6, lds_ld(R0 + c * SLOT));
       } else if (valid && gt_out) {
         for (int j = 0; j < 6; j++) gt_out[(size_t)it * 72 + c * 6 + j] = 0;
       }

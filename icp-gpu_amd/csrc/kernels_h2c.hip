@@ -6,13 +6,6 @@
 
 namespace bls {
 
-__device__ __forceinline__ void st_ref_h2c(u64* p, const Fp& a) {
-  u32 w[12];
-  to_ref(a, w);
-#pragma unroll
-  for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
-}
-
 // message i = msgs[offs[i] .. offs[i+1]) -> H(m) (or -H(m) when negate != 0) as an
 // affine point in reference form (12 u64: x then y); status[i] = 0, or 1 and zeros for
 // the identity (nullable).  Two lanes per message: both expand the message, lane 0 maps
@@ -42,8 +35,8 @@ __global__ __launch_bounds__(H2C_BLOCK) void k_hash_to_g1(const uint8_t* __restr
   const bool ok = clear_cofactor_affine(q, q1s[slot], x, y);
   u64* r = out + (size_t)i * 12;
   if (ok) {
-    st_ref_h2c(r, x);
-    st_ref_h2c(r + 6, negate ? neg(y) : y);
+    st_ref(r, x);
+    st_ref(r + 6, negate ? neg(y) : y);
   } else {
     for (int j = 0; j < 12; j++) r[j] = 0;
   }

@@ -130,12 +130,6 @@ __device__ __forceinline__ void ld_scalar(const uint8_t* b, u32 k[8]) {
   for (int j = 0; j < 8; j++)
     k[j] = (u32)b[4 * j] | ((u32)b[4 * j + 1] << 8) | ((u32)b[4 * j + 2] << 16) | ((u32)b[4 * j + 3] << 24);
 }
-__device__ __forceinline__ void st_ref_fp(u64* p, const Fp& a) {
-  u32 w[12];
-  to_ref(a, w);
-#pragma unroll
-  for (int j = 0; j < 6; j++) p[j] = (u64)w[2 * j] | ((u64)w[2 * j + 1] << 32);
-}
 
 __global__ void k_wl_keys(const uint8_t* __restrict__ sk, u64* __restrict__ pk, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,7 +138,7 @@ __global__ void k_wl_keys(const uint8_t* __restrict__ sk, u64* __restrict__ pk, 
   ld_scalar(sk + (size_t)i * 32, k);
   Aff<Fp2> q = to_affine(smul(g2_generator(), k, fp2_one()));
   u64* o = pk + (size_t)i * 24;
-  st_ref_fp(o, q.x.c0); st_ref_fp(o + 6, q.x.c1); st_ref_fp(o + 12, q.y.c0); st_ref_fp(o + 18, q.y.c1);
+  st_ref(o, q.x.c0); st_ref(o + 6, q.x.c1); st_ref(o + 12, q.y.c0); st_ref(o + 18, q.y.c1);
 }
 
 // one thread per item: H = hs*G1, nhm = -H, sig by category
@@ -157,8 +151,8 @@ __global__ void k_wl_items(const uint8_t* __restrict__ hs, const uint8_t* __rest
   u32 k[8], kk[8];
   ld_scalar(hs + (size_t)i * 32, k);
   Aff<Fp> H = to_affine(smul(g1_generator(), k, fp_one()));
-  st_ref_fp(nhm + (size_t)i * 12, H.x);
-  st_ref_fp(nhm + (size_t)i * 12 + 6, neg(H.y));
+  st_ref(nhm + (size_t)i * 12, H.x);
+  st_ref(nhm + (size_t)i * 12 + 6, neg(H.y));
   const int c = cat[i];
   u32 be[12];
   if (c == BLSGPU_WL_VALID || c == BLSGPU_WL_WRONG_MSG) {
@@ -206,13 +200,9 @@ __global__ void k_wl_sign_points(const u64* __restrict__ h_ref, const uint8_t* _
                                  const int32_t* __restrict__ key_idx, uint8_t* __restrict__ sig48, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  u32 w[12], kk[8];
+  u32 kk[8];
   const u64* h = h_ref + (size_t)i * 12;
-  Aff<Fp> H;
-  for (int j = 0; j < 6; j++) { w[2 * j] = (u32)h[j]; w[2 * j + 1] = (u32)(h[j] >> 32); }
-  H.x = from_ref(w);
-  for (int j = 0; j < 6; j++) { w[2 * j] = (u32)h[6 + j]; w[2 * j + 1] = (u32)(h[6 + j] >> 32); }
-  H.y = from_ref(w);
+  const Aff<Fp> H = {ld_ref(h), ld_ref(h + 6)};
   ld_scalar(sk + (size_t)key_idx[i] * 32, kk);
   u32 be[12];
   g1_compress(to_affine(smul(H, kk, fp_one())), false, be);

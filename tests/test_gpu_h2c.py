@@ -96,3 +96,6 @@ def test_rejects_bad_message_arguments():
         blsgpu.hash_to_g1([b"abc"], b"")  # the DST must be 1..255 bytes
     with pytest.raises(RuntimeError):
         blsgpu.hash_to_g1([b"abc"], bytes(256))
+    sig = np.zeros((1, 48), np.uint8)
+    with pytest.raises(RuntimeError):
+        blsgpu.verify_messages(sig, [b"abc"], b"", np.zeros(1, np.int32))
