@@ -90,3 +90,24 @@ def test_batched_and_serving_entry_points_validate_arguments():
 def test_server_needs_a_device():
     assert not blsgpu.load().blsgpu_server_create(2, 64, 1 << 20, 0, 0)
     assert "device" in blsgpu.last_error()
+
+
+def test_static_archive_links_a_caller(tmp_path):
+    # the same objects as libgpu_msm.a, for callers that link statically (the reference's
+    # Bazel route, BUILD.bazel:22-26): every declared symbol defined, and a C caller links
+    # against it with the HIP runtime and sees no device here
+    ar = os.path.join(ROOT, "icp-gpu_amd", "lib", "libgpu_msm.a")
+    out = subprocess.run(["nm", "-g", "--defined-only", ar], capture_output=True, text=True).stdout
+    defined = set(l.split()[-1] for l in out.splitlines() if " T " in l)
+    for s in declared_symbols():
+        assert s in defined, s
+    src = tmp_path / "caller.cpp"
+    src.write_text('#include <stdio.h>\n#include "gpu_msm.h"\n'
+                   'int main(void) { printf("%d\\n", gpu_crypto_init()); return 0; }\n')
+    exe = tmp_path / "caller"
+    subprocess.run(["g++", str(src), "-I", os.path.join(ROOT, "include"), ar, "-L/opt/rocm/lib", "-lamdhip64",
+                    "-Wl,-rpath,/opt/rocm/lib", "-lpthread", "-o", str(exe)], check=True, timeout=300)
+    if os.path.exists("/dev/kfd"):
+        return  # with a device the init would succeed; the link is what this test covers
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "-1"
