@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of each serving run (60 for the full runs)")
     ap.add_argument("--serve-rates", default="2000,20000,100000")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner for the batched forms")
+    ap.add_argument("--batched-engine", default="",
+                    help="lanes_miller:lanes_final_exp:wg_per_cu_miller:wg_per_cu_final_exp for the batched forms "
+                         "instead of the autotuner (e.g. 12:12:5:4)")
     return ap.parse_args()
 
 
@@ -221,7 +224,7 @@ def serving_pool(wl, seed, sk, n=65536):
     return dict(key_idx=key_idx, cat=cat, sig=sig, nhm=nhm)
 
 
-def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune):
+def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune, fixed=""):
     """plain form (one kernel sequence per batch) vs batched form (one graph-replayed
     sequence, queue-fed persistent kernels) on the same device-resident batches"""
     dev = torch.device("cuda:0")
@@ -236,7 +239,12 @@ def measure_batched(torch, lib, blsgpu, w, handles, steps, plain_steps, tune):
     sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     offp = off.ctypes.data_as(ctypes.c_void_p)
     tuned = None
-    if tune:
+    if fixed:
+        g = [int(v) for v in fixed.split(":")]
+        blsgpu.set_batched_engine(g[0], g[1], g[2], g[3])
+        tuned = {"lanes_miller": g[0], "lanes_final_exp": g[1], "wg_per_cu_miller": g[2], "wg_per_cu_final_exp": g[3],
+                 "fixed": True}
+    elif tune:
         cfg = np.zeros(4, np.int32)
         ms = np.zeros(2, np.float64)
         t_t = time.perf_counter()
@@ -545,7 +553,8 @@ def main():
         w3 = batched_workload(wl, args.seed + 3)
         lib.blsgpu_reset_keys()
         h3 = blsgpu.register_keys_affine(w3["pk"])
-        out["batched"] = measure_batched(torch, lib, blsgpu, w3, h3, args.batched_steps, 2, not args.no_tune)
+        out["batched"] = measure_batched(torch, lib, blsgpu, w3, h3, args.batched_steps, 2, not args.no_tune,
+                                         args.batched_engine)
         out["batched"]["workload"] = ("256 batches, geometric sizes mean 128 on [1,2048], 13 signers, 1024 messages "
                                       "each signed by 1..13 signers (%d signatures), 1/128 invalid"
                                       % w3["signatures"])
@@ -558,7 +567,7 @@ def main():
         h4 = blsgpu.register_keys_affine(w4["pk"])
     if "at_scale" in forms:
         out["at_scale"] = measure_batched(torch, lib, blsgpu, w4, h4, max(2, args.batched_steps // 2), 1,
-                                          not args.no_tune)
+                                          not args.no_tune, args.batched_engine)
         out["at_scale"]["workload"] = ("65536 verifications, 40 signer keys, 4096 messages, lognormal batch sizes "
                                        "(median 64, sigma 1.2, [1,8192]), 1/128 invalid")
         log("at_scale: %s" % {k: out["at_scale"][k] for k in ("batched_verif_per_s", "plain_verif_per_s", "speedup")})
