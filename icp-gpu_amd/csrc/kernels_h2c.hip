@@ -8,31 +8,35 @@ namespace bls {
 
 // message i = msgs[offs[i] .. offs[i+1]) -> H(m) (or -H(m) when negate != 0) as an
 // affine point in reference form (12 u64: x then y); status[i] = 0, or 1 and zeros for
-// the identity (nullable).  Two lanes per message: both expand the message, lane 0 maps
-// u0 and lane 1 u1 onto the curve at the same time; lane 1 hands its point over through
-// LDS and lane 0 adds, clears the cofactor and converts to affine.
+// the identity (nullable).  Two lanes per message (adjacent lanes of a wavefront): both
+// expand the message, lane 0 maps u0 and lane 1 u1 onto the curve at the same time, the
+// pair swaps its points across and both add them and clear the cofactor together (the
+// doublings split over the pair); lane 0 stores.
 constexpr int H2C_BLOCK = 64;
 __global__ __launch_bounds__(H2C_BLOCK) void k_hash_to_g1(const uint8_t* __restrict__ msgs,
                                                          const int64_t* __restrict__ offs, int n,
                                                          const uint8_t* __restrict__ dst, int dlen,
                                                          u64* __restrict__ out, i32* __restrict__ status,
                                                          int negate) {
-  __shared__ Jac<Fp> q1s[H2C_BLOCK / 2];
   const int t = blockIdx.x * H2C_BLOCK + threadIdx.x;
-  const int i = t >> 1, lane = t & 1, slot = threadIdx.x >> 1;
-  const bool active = i < n;
-  Jac<Fp> q;
-  if (active) {
-    const int64_t o = offs[i];
-    Fp u0, u1;
-    hash_to_field2(msgs + o, (int)(offs[i + 1] - o), dst, dlen, u0, u1);
-    q = map_to_g1_curve<FieldInline>(lane ? u1 : u0);
-    if (lane) q1s[slot] = q;
-  }
-  __syncthreads();
-  if (!active || lane) return;
+  const int i = t >> 1;
+  const bool odd = t & 1;
+  if (i >= n) return;  // both lanes of a pair leave together (n pairs, pairs aligned)
+  const int64_t o = offs[i];
+  Fp u0, u1;
+  hash_to_field2(msgs + o, (int)(offs[i + 1] - o), dst, dlen, u0, u1);
+  const Jac<Fp> q = map_to_g1_curve<FieldInline>(odd ? u1 : u0);
+  Jac<Fp> other;
+  other.x = pair_swap(q.x);
+  other.y = pair_swap(q.y);
+  other.z = pair_swap(q.z);
+  // Q0 and Q1 by selects, and one call: the pair must run the doublings together (a
+  // branch per lane parity would mask the partner off under every DPP read)
+  const Jac<Fp> q0 = {pick(odd, other.x, q.x), pick(odd, other.y, q.y), pick(odd, other.z, q.z)};
+  const Jac<Fp> q1 = {pick(odd, q.x, other.x), pick(odd, q.y, other.y), pick(odd, q.z, other.z)};
   Fp x, y;
-  const bool ok = clear_cofactor_affine(q, q1s[slot], x, y);
+  const bool ok = clear_cofactor_affine(q0, q1, odd, x, y);
+  if (odd) return;
   u64* r = out + (size_t)i * 12;
   if (ok) {
     st_ref(r, x);
