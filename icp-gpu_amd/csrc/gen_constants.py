@@ -184,7 +184,7 @@ def window4(e):
     return win
 
 
-def main():
+def main(out_path=None):
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
 
     def ref_fp(hexlist):  # reference R=2^384 Montgomery limbs -> canonical value
@@ -311,9 +311,9 @@ def main():
         emit("constexpr int32_t %s[%d][14] = {%s};" % (name, len(coeffs), ", ".join(
             "{%s}" % ", ".join("0x%07x" % x for x in limbs28(to_int(c))) for c in coeffs)))
     emit("}  // namespace bls")
-    with open(os.path.join(HERE, "field_constants.hpp"), "w") as f:
+    with open(out_path or os.path.join(HERE, "field_constants.hpp"), "w") as f:
         f.write("\n".join(out) + "\n")
-    print("wrote field_constants.hpp")
+    print("wrote", out_path or "field_constants.hpp")
 
 
 if __name__ == "__main__":

@@ -88,3 +88,13 @@ def test_device_isogeny_maps_equal_point_formula():
         want = h2c.iso_map((x, y))
         got = (ev(xn, x) * h2c.inv(ev(xd, x)) % P, y * ev(yn, x) * h2c.inv(ev(yd, x)) % P)
         assert got == want
+
+
+def test_constants_header_is_fresh(tmp_path):
+    # field_constants.hpp is what gen_constants.py derives (its asserts also check the
+    # reference's constants and the RFC's published isogeny coefficients)
+    import gen_constants as G
+    out = tmp_path / "field_constants.hpp"
+    G.main(str(out))
+    header = os.path.join(ROOT, "icp-gpu_amd", "csrc", "field_constants.hpp")
+    assert out.read_text() == open(header).read()
