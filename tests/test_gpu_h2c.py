@@ -99,3 +99,21 @@ def test_rejects_bad_message_arguments():
     sig = np.zeros((1, 48), np.uint8)
     with pytest.raises(RuntimeError):
         blsgpu.verify_messages(sig, [b"abc"], b"", np.zeros(1, np.int32))
+
+
+def test_large_batch_on_curve_and_sampled_against_oracle():
+    # 3,000 messages (94 workgroups of 32 lane pairs): every output on E and in the
+    # r-torsion ([r]P checked on a sample), a sample against the oracle
+    print("case: hash_to_g1 n=3000", flush=True)
+    n = 3000
+    msgs = [bytes(O.rand_bytes(97, 8_000_000 + i, 32)) for i in range(n)]
+    out, ident = blsgpu.hash_to_g1(msgs, h2c.IC_DST)
+    assert not ident.any()
+    pts = [_affine(row) for row in out]
+    for x, y in pts:
+        assert (y * y - x ** 3 - 4) % O.P == 0
+    assert len(set(pts)) == n
+    for i in list(range(0, n, 211)) + [n - 1]:
+        assert pts[i] == h2c.hash_to_g1(msgs[i], h2c.IC_DST), i
+    for i in (0, n // 2, n - 1):
+        assert h2c.ec_mul(pts[i], O.R_ORDER, 0) is None
