@@ -122,7 +122,7 @@ def cpu_baseline(sig, nhm, pk, key_idx, gpu_res, budget_s):
         h = np.ascontiguousarray(nhm[idx])
         res = np.zeros(len(idx), np.int32)
         t0 = time.perf_counter()
-        O.lib().orc_verify_compressed_batch(O.ptr(s), O.ptr(h), O.ptr(g2c), O.ptr(kc), O.ptr(ki), O.ptr(res),
+        O.lib().orc_verify_compressed_batch(O.ptr(s), O.ptr(h), O.ptr(g2c), O.ptr(kc), O.ptr(ki), O.ptr(res), None,
                                            len(idx), threads)
         return res, time.perf_counter() - t0
     _, t1 = run(np.arange(4), 1)

@@ -404,8 +404,14 @@ int blsgpu_num_keys(void) {
   std::lock_guard<std::mutex> lk(g.mu);
   return g.nkeys;
 }
+// The tables are retired, not reused: verifications already enqueued (device-form calls
+// on callers' streams, serving steps) may still read them, and the next registration
+// must not overwrite rows they read.  A retired buffer lives until the process ends.
 void blsgpu_reset_keys(void) {
   std::lock_guard<std::mutex> lk(g.mu);
+  if (g.keys.p) g.retired_keys.push_back(g.keys.p);
+  g.keys.p = nullptr;
+  g.keys.cap = 0;
   g.nkeys = 0;
 }
 

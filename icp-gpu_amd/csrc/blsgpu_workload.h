@@ -1,6 +1,8 @@
 /*
- * Synthetic workload generator and device probes of libgpu_msm.so.  Not part of the
- * reference ABI (no counterpart in the reference); used by bench.py and the tests.
+ * libblsgpu_workload.so: the synthetic workload generator and the multiply-add issue
+ * probe used by bench.py and the tests.  A library of its own, built beside
+ * libgpu_msm.so and not linked into it: the verifier a replica loads holds no key
+ * generator or signer.  No counterpart in the reference.
  * The seeded streams match the test oracle's generator, so the same (seed, index)
  * gives the same keys, messages and encodings.
  */

@@ -25,9 +25,11 @@ constexpr u32 SHA256_K[64] = {
 constexpr u32 SHA256_IV[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
                               0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
 
-__device__ __forceinline__ u32 rotr32(u32 x, int n) { return (x >> n) | (x << (32 - n)); }
+// one SHA-256 compression with a rolling 16-word schedule; host and device (the
+// synthetic workload generator hashes on the host with the same function)
+__host__ __device__ __forceinline__ u32 rotr32(u32 x, int n) { return (x >> n) | (x << (32 - n)); }
 
-__device__ __forceinline__ void sha256_compress(u32 (&h)[8], u32 (&w)[16]) {
+__host__ __device__ __forceinline__ void sha256_compress(u32 (&h)[8], u32 (&w)[16]) {
   u32 a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], k = h[7];
 #pragma unroll
   for (int i = 0; i < 64; i++) {

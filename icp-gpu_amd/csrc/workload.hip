@@ -1,4 +1,5 @@
-// Synthetic workload generator and device probes (declared in include/blsgpu_workload.h).
+// Synthetic workload generator and device probes of libblsgpu_workload.so (declared in
+// blsgpu_workload.h beside this file).
 //
 // Generates the benchmark configurations' inputs on the device: secret keys from a
 // seeded splitmix64 stream, pk = sk*G2, H(m) = s*G1 with s from SHA-256 as in the
@@ -13,8 +14,8 @@
 #include <string>
 #include <vector>
 
-#include "../../include/blsgpu_workload.h"
-#include "curve.hpp"
+#include "blsgpu_workload.h"
+#include "h2c.hpp"
 
 namespace bls {
 
@@ -55,39 +56,21 @@ static void rand_scalar(u64 seed, u64 index, uint8_t out[32]) {
   }
 }
 
-// ---------------- SHA-256 (FIPS 180-4) ----------------
-static const uint32_t K256[64] = {
-    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
-    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
-    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
-    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
-    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
-    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
-    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
-    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
-static inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+// SHA-256 on the host, through the compression function of the device hash (h2c.hpp)
 static void sha256(const uint8_t* msg, size_t len, uint8_t out[32]) {
-  uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a, 0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+  u32 h[8];
+  for (int k = 0; k < 8; k++) h[k] = SHA256_IV[k];
   std::vector<uint8_t> buf(msg, msg + len);
   buf.push_back(0x80);
   while (buf.size() % 64 != 56) buf.push_back(0);
-  uint64_t bits = (uint64_t)len * 8;
+  const uint64_t bits = (uint64_t)len * 8;
   for (int k = 7; k >= 0; k--) buf.push_back((uint8_t)(bits >> (8 * k)));
   for (size_t off = 0; off < buf.size(); off += 64) {
-    uint32_t w[64];
+    u32 w[16];
     for (int i = 0; i < 16; i++)
-      w[i] = (uint32_t)buf[off + 4 * i] << 24 | (uint32_t)buf[off + 4 * i + 1] << 16 |
-             (uint32_t)buf[off + 4 * i + 2] << 8 | buf[off + 4 * i + 3];
-    for (int i = 16; i < 64; i++)
-      w[i] = w[i - 16] + (rotr(w[i - 15], 7) ^ rotr(w[i - 15], 18) ^ (w[i - 15] >> 3)) + w[i - 7] +
-             (rotr(w[i - 2], 17) ^ rotr(w[i - 2], 19) ^ (w[i - 2] >> 10));
-    uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
-    for (int i = 0; i < 64; i++) {
-      uint32_t t1 = hh + (rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25)) + ((e & f) ^ (~e & g)) + K256[i] + w[i];
-      uint32_t t2 = (rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22)) + ((a & b) ^ (a & c) ^ (b & c));
-      hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
-    }
-    h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+      w[i] = (u32)buf[off + 4 * i] << 24 | (u32)buf[off + 4 * i + 1] << 16 | (u32)buf[off + 4 * i + 2] << 8 |
+             buf[off + 4 * i + 3];
+    sha256_compress(h, w);
   }
   for (int k = 0; k < 8; k++) {
     out[4 * k] = (uint8_t)(h[k] >> 24); out[4 * k + 1] = (uint8_t)(h[k] >> 16);

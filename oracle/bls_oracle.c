@@ -802,44 +802,50 @@ int orc_verify_batch_keyed(const u64* sigs, const u64* nhms, const u64* g2c, con
 }
 
 /* fused CPU path: decompress (with all checks) + verify against keyed lines.
- * results as the GPU's fused path: 1 valid, 0 invalid, negative decode code. */
+ * results as the GPU's fused path: 1 valid, 0 invalid, negative decode code; gts
+ * (nullable) receives each item's GT element, zeros for a rejected encoding.
+ * An identity signature (0xc0 encoding) is paired as the identity, as ic_bls12_381's
+ * multi_miller_loop does with an identity term (its line evaluations leave f as it is,
+ * oracle/src/main.rs:403-420): the Miller value is that of the (-H(m), pk) pair alone,
+ * so GT = e(-H(m), pk) and the verdict is 0 for any pk other than the identity. */
 typedef struct {
     const uint8_t* sig48; const u64 *nhms, *g2c, *key_coeffs; const int32_t* key_idx;
-    int32_t* res; int m, stride, tid;
+    int32_t* res; u64* gts; int m, stride, tid;
 } cjob_t;
 static void* cverify_worker(void* arg) {
     cjob_t* j = (cjob_t*)arg;
     for (int i = j->tid; i < j->m; i += j->stride) {
         u64 pt[12];
+        u64* gt = j->gts ? j->gts + (size_t)i * 72 : NULL;
+        const u64* pkc = j->key_coeffs + (size_t)j->key_idx[i] * 2448;
         int rc = orc_g1_decompress(j->sig48 + (size_t)i * 48, pt);
         if (rc == ORC_DEC_IDENTITY) {
-            /* identity signature: e(O, G2) = 1, so only the second pair remains */
-            fp12 ml, r;
-            LD(g1a, n, j->nhms + (size_t)i * 12);
-            const fp2(*pk)[3] = (const fp2(*)[3])(j->key_coeffs + (size_t)j->key_idx[i] * 2448);
-            u64 tmp[72];
-            orc_miller1(j->nhms + (size_t)i * 12, (const u64*)pk, tmp);
-            (void)n;
-            memcpy(&ml, tmp, sizeof ml);
-            final_exp(&r, &ml);
-            j->res[i] = fp12_is_one(&r);
+            u64 ml[72], r[72];
+            orc_miller1(j->nhms + (size_t)i * 12, pkc, ml);
+            orc_final_exp(ml, r);
+            LD(fp12, rv, r);
+            j->res[i] = fp12_is_one(&rv);
+            if (gt) memcpy(gt, r, sizeof r);
             continue;
         }
-        if (rc != ORC_DEC_OK) { j->res[i] = rc; continue; }
-        j->res[i] = orc_verify(pt, j->nhms + (size_t)i * 12, j->g2c,
-                               j->key_coeffs + (size_t)j->key_idx[i] * 2448, NULL);
+        if (rc != ORC_DEC_OK) {
+            j->res[i] = rc;
+            if (gt) memset(gt, 0, 72 * sizeof(u64));
+            continue;
+        }
+        j->res[i] = orc_verify(pt, j->nhms + (size_t)i * 12, j->g2c, pkc, gt);
     }
     return NULL;
 }
 int orc_verify_compressed_batch(const uint8_t* sig48, const u64* nhms, const u64* g2c, const u64* key_coeffs,
-                                const int32_t* key_idx, int32_t* results, int m, int threads) {
+                                const int32_t* key_idx, int32_t* results, u64* gts, int m, int threads) {
     if (threads <= 0) threads = orc_num_threads();
     if (threads > 256) threads = 256;
     if (threads > m) threads = m > 0 ? m : 1;
     pthread_t th[256];
     cjob_t jobs[256];
     for (int t = 0; t < threads; t++) {
-        cjob_t b = {sig48, nhms, g2c, key_coeffs, key_idx, results, m, threads, t};
+        cjob_t b = {sig48, nhms, g2c, key_coeffs, key_idx, results, gts, m, threads, t};
         jobs[t] = b;
         if (t > 0) pthread_create(&th[t], NULL, cverify_worker, &jobs[t]);
     }
@@ -902,25 +908,6 @@ int orc_g1_decompress(const uint8_t in[48], u64 out[12]) {
     LD(g1a, a, out);
     if (!g1a_in_subgroup(&a)) return ORC_DEC_NOT_IN_SUBGROUP;
     return ORC_DEC_OK;
-}
-/* decompress.cu:140-188 as shipped: no flag/range/sqrt/subgroup checks */
-void orc_g1_decompress_legacy(const uint8_t in[48], u64 out[12]) {
-    memset(out, 0, 96);
-    if ((in[0] >> 6) & 1) return;
-    int sflag = (in[0] >> 5) & 1;
-    uint8_t xb[48];
-    memcpy(xb, in, 48);
-    xb[0] &= 0x1f;
-    fp raw; fp_from_be_raw(&raw, xb);
-    g1a a;
-    fp_mul(&a.x, &raw, &R2); /* raw * R^2 * R^-1, reduced */
-    fp y2, b;
-    fp_sqr(&y2, &a.x); fp_mul(&y2, &y2, &a.x);
-    fp_add(&b, &ONE, &ONE); fp_add(&b, &b, &b);
-    fp_add(&y2, &y2, &b);
-    fp_sqrt(&a.y, &y2);
-    if (sflag != fp_lex_largest(&a.y)) fp_neg(&a.y, &a.y);
-    ST(out, a);
 }
 /* ---- G2 compression (ic_bls12_381 G2Affine::to_compressed / from_compressed) ----
  * 96 bytes: x.c1 then x.c0, each 48 big-endian bytes; the flags sit in the top three

@@ -81,10 +81,12 @@ int  orc_verify_batch_keyed(const uint64_t* sigs, const uint64_t* neg_hms, const
                             int32_t* results, uint64_t* gts, int m, int threads);
 
 /* fused CPU path: decompress each 48-byte signature with all checks, then verify;
- * results: 1 valid, 0 invalid, negative ORC_DEC_* code for a rejected encoding. */
+ * results: 1 valid, 0 invalid, negative ORC_DEC_* code for a rejected encoding;
+ * gts (nullable, m x 72): GT of each item, zeros for a rejected encoding.  An identity
+ * signature is paired as the identity: GT = e(neg_hm, pk), verdict GT == 1. */
 int  orc_verify_compressed_batch(const uint8_t* sig48, const uint64_t* neg_hms, const uint64_t* g2c,
                                  const uint64_t* key_coeffs, const int32_t* key_idx,
-                                 int32_t* results, int m, int threads);
+                                 int32_t* results, uint64_t* gts, int m, int threads);
 
 /* ---- groups (affine Montgomery; identity encoded by *inf / return code) ---- */
 #define ORC_DEC_OK 0
@@ -113,8 +115,6 @@ int  orc_g2_is_on_curve(const uint64_t q[24]);
 void orc_g1_neg(const uint64_t p[12], uint64_t out[12]);
 void orc_g1_generator(uint64_t out[12]);
 void orc_g2_generator(uint64_t out[24]);
-/* legacy decompression semantics of decompress.cu:140-188 (no validity checks) */
-void orc_g1_decompress_legacy(const uint8_t in[48], uint64_t out[12]);
 
 /* ---- hashing and workload ---- */
 void orc_sha256(const uint8_t* msg, size_t len, uint8_t out[32]);

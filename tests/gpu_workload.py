@@ -7,12 +7,11 @@ import numpy as np
 
 import blsgpu
 import oracle as O
+from blsgpu import workload
 
 
 def wl_lib():
-    lib = ctypes.CDLL(blsgpu.LIB_PATH)
-    lib.blsgpu_wl_last_error.restype = ctypes.c_char_p
-    return lib
+    return workload.lib()
 
 
 def make_workload(seed, nkeys, n, n_msgs, invalid_every=0):
@@ -39,26 +38,9 @@ def make_workload(seed, nkeys, n, n_msgs, invalid_every=0):
 
 
 def oracle_expected(sig48, nhm, pks, key_idx, threads=0):
-    """Verdicts (1/0 or the negative decode code) and GT elements (zeros for rejected
-    encodings) of the oracle's fused path."""
-    n = sig48.shape[0]
+    """Verdicts (1/0 or the negative decode code) and GT elements of the oracle's fused
+    path (zeros for rejected encodings; an identity signature is paired as the identity,
+    so its GT is e(-H(m), pk) and its verdict 0)."""
     coeffs = np.array([O.g2_prepare(p) for p in pks])
     g2c = O.g2_prepare(O.g2_gen())
-    res = np.zeros(n, np.int32)
-    gts = np.zeros((n, 72), np.uint64)
-    unc = np.zeros((n, 12), np.uint64)
-    ok = np.zeros(n, bool)
-    for i in range(n):
-        rc, pt = O.g1_decompress(sig48[i])
-        if rc == 0:
-            unc[i] = pt
-            ok[i] = True
-        else:
-            res[i] = rc
-    idx = np.nonzero(ok)[0]
-    if len(idx):
-        r, g = O.verify_batch_keyed(np.ascontiguousarray(unc[idx]), np.ascontiguousarray(nhm[idx]), g2c, coeffs,
-                                    np.ascontiguousarray(key_idx[idx]), threads=threads)
-        res[idx] = r
-        gts[idx] = g
-    return res, gts
+    return O.verify_compressed_batch(sig48, nhm, g2c, coeffs, key_idx, threads=threads)

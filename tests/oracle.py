@@ -201,6 +201,25 @@ def verify_batch_keyed(sigs, nhms, g2c, key_coeffs, key_idx, threads=0, want_gt=
     return res, gts
 
 
+def verify_compressed_batch(sig48, nhms, g2c, key_coeffs, key_idx, threads=0, want_gt=True):
+    """The oracle's fused path: decode each 48-byte signature with every check, then
+    verify against the keyed lines.  Results 1/0 or the negative decode code; GT of
+    each item (zeros for a rejected encoding; e(-H(m), pk) for an identity signature)."""
+    m = sig48.shape[0]
+    res = np.zeros(m, np.int32)
+    gts = np.zeros((m, 72), np.uint64) if want_gt else None
+    sig48 = np.ascontiguousarray(sig48, dtype=np.uint8)
+    nhms = np.ascontiguousarray(nhms, dtype=np.uint64)
+    key_coeffs = np.ascontiguousarray(key_coeffs, dtype=np.uint64)
+    key_idx = np.ascontiguousarray(key_idx, dtype=np.int32)
+    lib().orc_verify_compressed_batch(ptr(sig48), ptr(nhms), ptr(np.ascontiguousarray(g2c)), ptr(key_coeffs),
+                                      ptr(key_idx), ptr(res), ptr(gts), m, threads)
+    return res, gts
+
+
+IDENTITY_G1 = np.array([0xC0] + [0] * 47, np.uint8)  # compressed encoding of the G1 identity
+
+
 # ---- seeded workload generation (the configs' shapes, small sizes for tests) ----
 def make_keys(seed, n):
     """n key pairs: sk, pk (affine G2, 24 u64) and pk lines (2448 u64)."""

@@ -111,3 +111,20 @@ def test_static_archive_links_a_caller(tmp_path):
         return  # with a device the init would succeed; the link is what this test covers
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "-1"
+
+
+def test_workload_generator_is_a_separate_library():
+    """The synthetic workload generator (keys, signer) and the issue-rate probe live in
+    libblsgpu_workload.so, which the benchmark and the tests load; libgpu_msm.so, the
+    library a replica loads, exports none of them."""
+    from blsgpu import workload
+    hdr = os.path.join(ROOT, "icp-gpu_amd", "csrc", "blsgpu_workload.h")
+    text = re.sub(r"/\*.*?\*/", "", open(hdr).read(), flags=re.S)
+    wl_syms = set(re.findall(r"\b(blsgpu_\w+)\s*\(", text))
+    assert "blsgpu_wl_items_msg" in wl_syms and "blsgpu_probe_mad_rate" in wl_syms
+    nm = lambda p: set(l.split()[-1] for l in subprocess.run(["nm", "-D", "--defined-only", p], capture_output=True,
+                                                               text=True).stdout.splitlines() if l.strip())
+    main, wl = nm(blsgpu.LIB_PATH), nm(workload.WORKLOAD_LIB_PATH)
+    assert wl_syms <= wl
+    assert not (wl_syms & main)
+    assert not any(s.startswith("blsgpu_wl_") for s in main)

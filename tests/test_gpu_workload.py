@@ -7,14 +7,13 @@ import pytest
 
 import blsgpu
 import oracle as O
+from blsgpu import workload
 
 pytestmark = pytest.mark.gpu
 
 
 def _wl():
-    lib = ctypes.CDLL(blsgpu.LIB_PATH)
-    lib.blsgpu_wl_last_error.restype = ctypes.c_char_p
-    return lib
+    return workload.lib()
 
 
 def test_generator_matches_oracle_streams():

@@ -231,3 +231,44 @@ def test_signature_scheme_end_to_end():
     assert ok
     ok, _ = O.verify(s, O.g1_neg(h), g2c, coeffs[1])
     assert not ok
+
+
+def test_identity_signature_is_paired_as_the_identity():
+    """The fused path's identity branch (a 0xc0 signature).  ic_bls12_381's
+    multi_miller_loop leaves f unchanged for an identity term, and e(O, G2) = 1, so the
+    item's GT must be e(-H(m), pk) alone, and (bilinearity, with sig = sk H(m) the valid
+    signature) also the inverse, i.e. the conjugate, of e(sig, G2).  A branch that paired
+    the identity as the flagless point (0, 0) or (0, 1) (the legacy marshalling drops the
+    flag) would multiply f by the lines' constant terms and fail both checks."""
+    sks, pks, coeffs = O.make_keys(11, 3)
+    g2c = O.g2_prepare(O.g2_gen())
+    comp, unc, nhm, msgs = O.make_batch(11, 6, sks, [0, 1, 2, 0, 1, 2])
+    ident = [0, 3, 5]  # first, middle and last items take the identity signature
+    for i in ident:
+        comp[i] = O.IDENTITY_G1
+    key_idx = np.array([0, 1, 2, 0, 1, 2], np.int32)
+    res, gts = O.verify_compressed_batch(comp, nhm, g2c, coeffs, key_idx, threads=2)
+    one = np.zeros(72, np.uint64)
+    one[:6] = O.mont(1)
+    for i in range(6):
+        k = int(key_idx[i])
+        if i in ident:
+            assert res[i] == 0, i
+            assert (gts[i] == _pairing(nhm[i], pks[k])).all(), i
+            assert not (gts[i] == one).all()
+            conj = np.zeros(72, np.uint64)
+            O.lib().orc_fp12_conj(O.ptr(_pairing(unc[i], O.g2_gen())), O.ptr(conj))
+            assert (gts[i] == conj).all(), i
+        else:
+            # the other items of the batch keep the two-pair result: valid, GT = 1
+            assert res[i] == 1 and (gts[i] == one).all(), i
+    # an identity signature fails against every key, whatever the message
+    res2, _ = O.verify_compressed_batch(np.stack([O.IDENTITY_G1] * 3), nhm[:3], g2c, coeffs,
+                                        np.arange(3, dtype=np.int32), threads=1)
+    assert res2.tolist() == [0, 0, 0]
+    # rejected encodings export zeros
+    bad = np.stack([O.IDENTITY_G1, O.IDENTITY_G1])
+    bad[1, 0] = 0xE0  # identity with the sort flag: bad flags
+    res3, gts3 = O.verify_compressed_batch(bad, nhm[:2], g2c, coeffs, np.zeros(2, np.int32), threads=1)
+    assert res3.tolist() == [0, -1]
+    assert (gts3[1] == 0).all()
