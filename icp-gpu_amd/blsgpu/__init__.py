@@ -38,6 +38,7 @@ EXTENSION_SYMBOLS = (
     "blsgpu_set_engine",
     "blsgpu_get_engine",
     "blsgpu_verify_compressed_device",
+    "blsgpu_verify_compressed_device_noalloc",
     "blsgpu_set_profiling",
     "blsgpu_phase_times",
     "blsgpu_register_keys_affine",
@@ -104,6 +105,7 @@ def load(path=None):
         "blsgpu_set_engine": ([i32, i32, i32], i32),
         "blsgpu_get_engine": ([vp, vp, vp], i32),
         "blsgpu_verify_compressed_device": ([vp, vp, vp, vp, vp, i32, vp], i32),
+        "blsgpu_verify_compressed_device_noalloc": ([vp, vp, vp, vp, vp, i32, vp], i32),
         "blsgpu_set_profiling": ([i32], i32),
         "blsgpu_phase_times": ([vp, vp], i32),
         "blsgpu_register_keys_affine": ([vp, i32, vp], i32),

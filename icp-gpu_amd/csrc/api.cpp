@@ -39,6 +39,29 @@ const uint64_t G2_GEN_REF[24] = {
 
 }  // namespace
 
+// Fixed workspace of blsgpu_verify_compressed_device_noalloc, for callers that cannot
+// allocate: per item the decoded signature and -H(m) (28 words each), the decode and
+// subgroup statuses and the Miller value (168 words), each part laid out for
+// BLSGPU_NOALLOC_MAX_ITEMS items at a 256-byte aligned offset.  Every word a call reads
+// is written earlier in the same call (decode writes every status, the subgroup test
+// every subgroup status, the Miller loop the value of every item the final
+// exponentiation reads), so nothing is cleared between calls.  In host code `scratch`
+// names the host-side shadow of the array, not device memory: host code uses only the
+// address hipGetSymbolAddress gives at init (g.scratch).
+namespace {
+constexpr size_t NA_MAX = BLSGPU_NOALLOC_MAX_ITEMS;
+constexpr size_t NA_SIG = 0;
+constexpr size_t NA_NHM = NA_SIG + NA_MAX * bls::POINT_WORDS * 4;
+constexpr size_t NA_STATUS = NA_NHM + NA_MAX * bls::POINT_WORDS * 4;
+constexpr size_t NA_SUB = NA_STATUS + NA_MAX * 4;
+constexpr size_t NA_FBUF = NA_SUB + NA_MAX * 4;
+constexpr size_t NA_BYTES = NA_FBUF + NA_MAX * bls::MILLER_WORDS * 4;
+static_assert(NA_NHM % 256 == 0 && NA_STATUS % 256 == 0 && NA_SUB % 256 == 0 && NA_FBUF % 256 == 0,
+              "aligned parts");
+static_assert(NA_BYTES == BLSGPU_NOALLOC_SCRATCH_BYTES, "the size the header states");
+}  // namespace
+static __device__ __align__(256) uint8_t scratch[NA_BYTES];
+
 // Small host arrays of a blocking call travel through one pinned staging area: one CPU
 // copy, then DMA from pinned memory (the runtime stages pageable copies itself, at a
 // fixed cost per copy that dominated these calls).  Every such call synchronizes before
@@ -119,6 +142,15 @@ int init_locked() {
   HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&g.ev_h2c, hipEventDisableTiming));
+  // the fixed workspace's device address on this device (g.device is current)
+  void* sp = nullptr;
+  HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  if (!sp) {
+    g_err = "no device address for the fixed workspace";
+    return -1;
+  }
+  g.scratch = sp;
+  HIPCHK(hipEventCreateWithFlags(&g.ev_scratch, hipEventDisableTiming));
   // generator lines, once
   CHK(g.g2_ref.ensure(24 * 8));
   CHK(g.gen_lines.ensure(bls::KEY_TABLE_WORDS * 4));
@@ -620,6 +652,56 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
   }
   CHK(bls::engine_compressed(a, s));
   if (g.profiling) g.ev_pending.push_back(ev);
+  return 0;
+}
+
+int blsgpu_verify_compressed_device_noalloc(const uint8_t* d_sigs48, const uint64_t* d_neg_hms,
+                                            const int32_t* d_key_idx, int32_t* d_results, uint64_t* d_gt, int M,
+                                            void* stream) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (M <= 0 || !d_sigs48 || !d_neg_hms || !d_key_idx || !d_results) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  if (M > BLSGPU_NOALLOC_MAX_ITEMS) {
+    g_err = "M exceeds BLSGPU_NOALLOC_MAX_ITEMS (8192 items), the capacity of the fixed workspace";
+    return -1;
+  }
+  // set-up (which allocates) belongs to gpu_crypto_init, never to this entry point
+  if (g.state != 1 || !g.scratch) {
+    g_err = "library not initialized: call gpu_crypto_init() first";
+    return -1;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : g.s;
+  // the previous call's kernels on the array finish first: stream order on the same
+  // stream, an event wait across streams (no host wait)
+  if (g.scratch_last && g.scratch_last != s) HIPCHK(hipStreamWaitEvent(s, g.ev_scratch, 0));
+  uint8_t* base = static_cast<uint8_t*>(g.scratch);
+  bls::EngineArgs a{};
+  a.M = M;
+  a.mode = g.mode;
+  a.g_ml = g.g_ml;
+  a.g_fe = g.g_fe;
+  a.comp = d_sigs48;
+  a.nhm_ref = d_neg_hms;
+  a.key_idx = d_key_idx;
+  a.keys = g.keys.as<int32_t>();
+  a.nkeys = g.nkeys;
+  a.gen_lines = g.gen_lines.as<int32_t>();
+  a.sig_pts = reinterpret_cast<int32_t*>(base + NA_SIG);
+  a.nhm_pts = reinterpret_cast<int32_t*>(base + NA_NHM);
+  a.status = reinterpret_cast<int32_t*>(base + NA_STATUS);
+  a.sub_status = reinterpret_cast<int32_t*>(base + NA_SUB);
+  a.fbuf = reinterpret_cast<int32_t*>(base + NA_FBUF);
+  a.s2 = g.s2;
+  a.fork = g.ev_fork;
+  a.join = g.ev_join;
+  a.results = d_results;
+  a.gt = d_gt;
+  CHK(bls::engine_compressed(a, s));
+  HIPCHK(hipEventRecord(g.ev_scratch, s));
+  g.scratch_last = s;
   return 0;
 }
 

@@ -102,6 +102,12 @@ struct Ctx {
   DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, sub_status, fbuf, res, gt;
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_h2c = nullptr;
+  // fixed workspace of blsgpu_verify_compressed_device_noalloc: the device address of the
+  // file-scope array in api.cpp on g.device (from hipGetSymbolAddress at init), the event
+  // recorded after the last call that used it and the stream that call ran on
+  void* scratch = nullptr;
+  hipEvent_t ev_scratch = nullptr;
+  hipStream_t scratch_last = nullptr;
   // resident
   DevBuf gen_lines;  // G2 generator lines (internal), computed at init
   DevBuf keys;       // registered key tables (internal, key-major)

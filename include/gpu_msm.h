@@ -133,6 +133,24 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
                                     const int32_t* d_key_idx, int32_t* d_results, uint64_t* d_gt,
                                     int M, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * blsgpu_verify_compressed_device (same arguments, results and GT) with the library's
+ * grow-only workspace replaced by a fixed device array of BLSGPU_NOALLOC_SCRATCH_BYTES
+ * (7,405,568 bytes: 904 per item for BLSGPU_NOALLOC_MAX_ITEMS items) that each GPU the
+ * library runs on holds until the process ends.  It calls no hipMalloc, hipFree or other
+ * allocating routine; set-up is gpu_crypto_init's (called first, else -1).  M above
+ * BLSGPU_NOALLOC_MAX_ITEMS returns -1 with a message.  Every call on a GPU shares that
+ * GPU's array: the library orders its own calls (a call on another stream waits, on the
+ * device, for the previous call's kernels), but work that overlaps on one GPU outside
+ * that order races on the array, e.g. a captured graph of this call replayed beside
+ * other calls of it. */
+#define BLSGPU_NOALLOC_MAX_ITEMS 8192
+#define BLSGPU_NOALLOC_SCRATCH_BYTES 7405568
+int blsgpu_verify_compressed_device_noalloc(const uint8_t* d_sigs48, const uint64_t* d_neg_hms,
+                                            const int32_t* d_key_idx, int32_t* d_results, uint64_t* d_gt,
+                                            int M, void* stream);
+
 /* Batched form: nbatches verify batches in one submission, with the same per-item
  * semantics as blsgpu_verify_compressed.  The Miller-loop and final-exponentiation
  * kernels are persistent (a device-sized grid pops tiles of items from a device work
