@@ -41,6 +41,7 @@ EXTENSION_SYMBOLS = (
     "blsgpu_verify_compressed_device_noalloc",
     "blsgpu_set_profiling",
     "blsgpu_phase_times",
+    "blsgpu_legacy_phase_times",
     "blsgpu_register_keys_affine",
     "blsgpu_register_keys_compressed",
     "blsgpu_hash_to_g1",
@@ -108,6 +109,7 @@ def load(path=None):
         "blsgpu_verify_compressed_device_noalloc": ([vp, vp, vp, vp, vp, i32, vp], i32),
         "blsgpu_set_profiling": ([i32], i32),
         "blsgpu_phase_times": ([vp, vp], i32),
+        "blsgpu_legacy_phase_times": ([vp, vp], i32),
         "blsgpu_register_keys_affine": ([vp, i32, vp], i32),
         "blsgpu_register_keys_compressed": ([vp, i32, vp, vp, i32], i32),
         "blsgpu_hash_to_g1": ([vp, vp, i32, vp, i32, vp, vp, i32], i32),

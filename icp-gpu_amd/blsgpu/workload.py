@@ -1,5 +1,5 @@
 """Seeded synthetic workloads of the benchmark configurations, generated on the device
-by libblsgpu_workload.so (include/blsgpu_workload.h).
+by libblsgpu_workload.so (declared in csrc/blsgpu_workload.h).
 
 That library is bench and test infrastructure: it is built beside libgpu_msm.so but is
 not part of it, so a replica that loads the verifier loads no key generator or signer.
@@ -197,3 +197,39 @@ def probe_mad_rate():
     if lib().blsgpu_probe_mad_rate(ctypes.byref(mads), ctypes.byref(ms)):
         raise _err("probe")
     return mads.value, ms.value
+
+
+_P = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+_R_INV = pow(1 << 384, -1, _P)
+
+
+def _canon(limbs):
+    return sum(int(v) << (64 * i) for i, v in enumerate(limbs)) * _R_INV % _P
+
+
+def g2_compress(pks):
+    """96-byte compressed encodings (ZCash format, as ic_bls12_381's G2Affine::
+    to_compressed: x.c1 then x.c0 big-endian, compression flag, sort flag when y is the
+    lexicographically largest root) of affine G2 points in reference form (n x 24 u64:
+    x.c0 x.c1 y.c0 y.c1, Montgomery, R = 2^384)"""
+    pks = np.asarray(pks, np.uint64).reshape(-1, 24)
+    out = np.zeros((pks.shape[0], 96), np.uint8)
+    half = (_P - 1) // 2
+    for i, q in enumerate(pks):
+        xc0, xc1, yc0, yc1 = (_canon(q[6 * k:6 * k + 6]) for k in range(4))
+        b = bytearray(xc1.to_bytes(48, "big") + xc0.to_bytes(48, "big"))
+        b[0] |= 0x80
+        if yc1 > half or (yc1 == 0 and yc0 > half):
+            b[0] |= 0x20
+        out[i] = np.frombuffer(bytes(b), np.uint8)
+    return out
+
+
+def g2_generator():
+    """the G2 generator in reference form (24 u64), as the public key of secret key 1"""
+    sk = np.zeros((1, 32), np.uint8)
+    sk[0, 0] = 1
+    pk = np.zeros((1, 24), np.uint64)
+    if lib().blsgpu_wl_public_keys(_p(sk), 1, _p(pk)):
+        raise _err("public keys")
+    return pk[0]

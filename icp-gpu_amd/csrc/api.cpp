@@ -180,10 +180,24 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   Staging st;
   CHK(st.begin(2 * Staging::span(m * 96) + Staging::span(2448 * 8) + Staging::span(m * 2448 * 8) +
                Staging::span(m * 4) + (gt_out ? Staging::span(m * 72 * 8) : 0)));
+  // with profiling on: events around the copies in, the kernels and the copies out
+  hipEvent_t ev[4] = {};
+  if (g.profiling) {
+    for (hipEvent_t& e : ev) {
+      if (!g.ev_free.empty()) {
+        e = g.ev_free.back();
+        g.ev_free.pop_back();
+      } else {
+        HIPCHK(hipEventCreate(&e));
+      }
+    }
+    HIPCHK(hipEventRecord(ev[0], g.s));
+  }
   CHK(st.h2d(g.sig_ref.p, sigs, m * 96, g.s));
   CHK(st.h2d(g.nhm_ref.p, nhms, m * 96, g.s));
   CHK(st.h2d(g.g2_ref.p, g2c, 2448 * 8, g.s));
   CHK(st.h2d(g.pk_ref.p, pkc, m * 2448 * 8, g.s));
+  if (ev[1]) HIPCHK(hipEventRecord(ev[1], g.s));
   bls::EngineArgs a{};
   a.M = M;
   a.mode = g.mode;
@@ -201,10 +215,21 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   a.results = g.res.as<int32_t>();
   a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
   CHK(bls::engine_legacy(a, g.s));
+  if (ev[2]) HIPCHK(hipEventRecord(ev[2], g.s));
   CHK(st.d2h(results, g.res.p, m * 4, g.s));
   if (gt_out) CHK(st.d2h(gt_out, g.gt.p, m * 72 * 8, g.s));
+  if (ev[3]) HIPCHK(hipEventRecord(ev[3], g.s));
   HIPCHK(hipStreamSynchronize(g.s));
   st.finish();
+  if (ev[0]) {
+    for (int k = 0; k < 3; k++) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      g.legacy_ms[k] += t;
+    }
+    g.legacy_calls++;
+    for (hipEvent_t e : ev) g.ev_free.push_back(e);
+  }
   return 0;
 }
 
@@ -734,6 +759,15 @@ int blsgpu_set_profiling(int on) {
   g.profiling = on != 0;
   for (int k = 0; k < 4; k++) g.phase_ms[k] = 0;
   g.phase_calls = 0;
+  for (int k = 0; k < 3; k++) g.legacy_ms[k] = 0;
+  g.legacy_calls = 0;
+  return 0;
+}
+
+int blsgpu_legacy_phase_times(double ms[3], int* calls) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  for (int k = 0; k < 3; k++) ms[k] = g.legacy_ms[k];
+  if (calls) *calls = g.legacy_calls;
   return 0;
 }
 

@@ -122,6 +122,9 @@ struct Ctx {
   std::vector<std::vector<hipEvent_t>> ev_pending;
   double phase_ms[4] = {0, 0, 0, 0};
   int phase_calls = 0;
+  // the legacy entry point's host-to-device copies, kernels and device-to-host copies
+  double legacy_ms[3] = {0, 0, 0};
+  int legacy_calls = 0;
   // engine configuration of the plain form (blsgpu_set_engine)
   int mode = 1, g_ml = 12, g_fe = 16;
   // batched (queue-fed) form: its own workspace, the configuration by batch-size class

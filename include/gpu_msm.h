@@ -209,6 +209,10 @@ int blsgpu_get_engine(int* mode, int* g_miller, int* g_final_exp);
  * 3 final exponentiation.  ms[k] = summed milliseconds; *calls = calls accumulated. */
 int blsgpu_set_profiling(int on);
 int blsgpu_phase_times(double ms[4], int* calls);
+/* The same for gpu_batch_bls_verify / blsgpu_batch_verify_gt calls made while profiling
+ * is on: ms[0] host-to-device copies of the caller's arrays, ms[1] the kernels, ms[2]
+ * the device-to-host copies of the results, summed over *calls calls. */
+int blsgpu_legacy_phase_times(double ms[3], int* calls);
 
 /* Last error message of the calling thread ("" if none). */
 const char* blsgpu_last_error(void);
