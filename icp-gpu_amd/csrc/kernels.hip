@@ -98,7 +98,7 @@ __global__ void k_lines_from_ref(const u64* __restrict__ in, i32* __restrict__ o
 
 // G2Prepared of K affine points given in reference form (24 u64: x.c0 x.c1 y.c0 y.c1)
 // -> key-major internal tables (K x 5712 words)
-__global__ void k_g2_prepare(const u64* __restrict__ q, i32* __restrict__ out, int K) {
+__global__ __launch_bounds__(64, 1) void k_g2_prepare(const u64* __restrict__ q, i32* __restrict__ out, int K) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= K) return;
   const u64* s = q + (size_t)i * 24;
@@ -114,7 +114,7 @@ __global__ void k_g2_prepare(const u64* __restrict__ q, i32* __restrict__ out, i
 // compressed G2 public keys (K x 96 bytes) -> affine points in reference form (K x 24
 // u64: x.c0 x.c1 y.c0 y.c1; zero where the encoding is rejected) + status per key,
 // with the subgroup test (check_subgroup != 0).  One thread per key.
-__global__ void k_g2_decompress(const uint8_t* __restrict__ in, u64* __restrict__ q_ref, i32* __restrict__ status,
+__global__ __launch_bounds__(64, 1) void k_g2_decompress(const uint8_t* __restrict__ in, u64* __restrict__ q_ref, i32* __restrict__ status,
                                 int K, int check_subgroup) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= K) return;
@@ -138,7 +138,7 @@ __global__ void k_g2_decompress(const uint8_t* __restrict__ in, u64* __restrict_
 }
 
 // compressed G1 (M x 48 bytes) -> internal SoA points + status per item
-__global__ void k_g1_decompress(const uint8_t* __restrict__ in, i32* __restrict__ pts, i32* __restrict__ status,
+__global__ __launch_bounds__(64, 1) void k_g1_decompress(const uint8_t* __restrict__ in, i32* __restrict__ pts, i32* __restrict__ status,
                                 int M, int check_subgroup) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;

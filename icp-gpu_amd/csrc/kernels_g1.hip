@@ -17,7 +17,7 @@ __device__ __forceinline__ void st_soa(i32* base, int w0, int M, int i, const Fp
   for (int j = 0; j < NL; j++) base[(size_t)(w0 + j) * M + i] = v.l[j];
 }
 
-__global__ __launch_bounds__(64) void k_g1_decode(const uint8_t* __restrict__ in, i32* __restrict__ pts,
+__global__ __launch_bounds__(64, 1) void k_g1_decode(const uint8_t* __restrict__ in, i32* __restrict__ pts,
                                                   i32* __restrict__ status, int M) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(64) void k_g1_decode(const uint8_t* __restrict__ in
 }
 
 // one thread per item (g1_subgroup_status)
-__global__ __launch_bounds__(64) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
+__global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
                                                     i32* __restrict__ sub_status, int M) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;

@@ -820,6 +820,15 @@ static int queue_grid(int occ, int ntiles, int wg_per_cu) {
   return (int)(g < ntiles ? g : ntiles);
 }
 
+// Whether a launch puts at most one wave on each SIMD when its waves are spread evenly:
+// one workgroup per tile (M up to a few thousand items), or a persistent grid of at most
+// four single-wave workgroups per CU.
+static bool at_most_one_wave_per_simd(int* queue, int occ, int wg_per_cu, int waves_per_wg) {
+  if (!queue) return true;
+  const int per = wg_per_cu > 0 && wg_per_cu < occ ? wg_per_cu : occ;
+  return per * waves_per_wg <= 4;
+}
+
 // grid of a launch: the persistent grid with a queue, else one workgroup per tile plus
 // the subgroup-test workgroups (riders with per <= threads of a workgroup, no queue)
 static int launch_grid(int occ, int ntiles, int threads, int* queue, int wg_per_cu, const SubgroupRiders& r, int M,
@@ -842,9 +851,15 @@ int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, co
   const int ntiles = (M + IPW - 1) / IPW;
   int grid = 0;
   if (riders.per && !rides<G>(0)) return -1;
-  if (launch_grid(queue ? grp_occ_ml<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  const int occ = queue ? grp_occ_ml<G>() : 0;
+  if (launch_grid(occ, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  // At most one wave per SIMD wanted: the instantiation compiled for one wave per SIMD
+  // where there is one (its registers exceed half a SIMD's, so no SIMD takes two of its
+  // waves while another SIMD of the CU is empty; with the two-wave instantiation the
+  // dispatcher doubles waves up, which measured 2.47 instead of 1.47 ms for the 12-lane
+  // final exponentiation of 4096 items).
   if constexpr (rides<G>(0)) {
-    if (riders.per) {
+    if (riders.per || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W)) {
       hipLaunchKernelGGL((k_miller_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen,
                          keys, key_idx, nkeys, pk_soa, status, fout, M, queue, riders);
       return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -861,9 +876,10 @@ int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_
   const int ntiles = (M + IPW - 1) / IPW;
   int grid = 0;
   if (riders.per && !rides<G>(1)) return -1;
-  if (launch_grid(queue ? grp_occ_fe<G>() : 0, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
+  const int occ = queue ? grp_occ_fe<G>() : 0;
+  if (launch_grid(occ, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
   if constexpr (rides<G>(1)) {
-    if (riders.per) {
+    if (riders.per || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W)) {
       hipLaunchKernelGGL((k_final_exp_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), final_exp_lds_bytes<G>(), s, fin,
                          status, sub_status, result, gt, M, queue, riders);
       return hipGetLastError() == hipSuccess ? 0 : -1;

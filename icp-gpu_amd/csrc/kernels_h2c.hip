@@ -13,7 +13,7 @@ namespace bls {
 // pair swaps its points across and both add them and clear the cofactor together (the
 // doublings split over the pair); lane 0 stores.
 constexpr int H2C_BLOCK = 64;
-__global__ __launch_bounds__(H2C_BLOCK) void k_hash_to_g1(const uint8_t* __restrict__ msgs,
+__global__ __launch_bounds__(H2C_BLOCK, 1) void k_hash_to_g1(const uint8_t* __restrict__ msgs,
                                                          const int64_t* __restrict__ offs, int n,
                                                          const uint8_t* __restrict__ dst, int dlen,
                                                          u64* __restrict__ out, i32* __restrict__ status,

@@ -114,7 +114,7 @@ __device__ __forceinline__ void ld_scalar(const uint8_t* b, u32 k[8]) {
     k[j] = (u32)b[4 * j] | ((u32)b[4 * j + 1] << 8) | ((u32)b[4 * j + 2] << 16) | ((u32)b[4 * j + 3] << 24);
 }
 
-__global__ void k_wl_keys(const uint8_t* __restrict__ sk, u64* __restrict__ pk, int n) {
+__global__ __launch_bounds__(64, 1) void k_wl_keys(const uint8_t* __restrict__ sk, u64* __restrict__ pk, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   u32 k[8];
@@ -125,7 +125,7 @@ __global__ void k_wl_keys(const uint8_t* __restrict__ sk, u64* __restrict__ pk, 
 }
 
 // one thread per item: H = hs*G1, nhm = -H, sig by category
-__global__ void k_wl_items(const uint8_t* __restrict__ hs, const uint8_t* __restrict__ hs_other,
+__global__ __launch_bounds__(64, 1) void k_wl_items(const uint8_t* __restrict__ hs, const uint8_t* __restrict__ hs_other,
                            const uint8_t* __restrict__ sk, const int32_t* __restrict__ key_idx,
                            const int8_t* __restrict__ cat, u64 seed, uint8_t* __restrict__ sig48,
                            u64* __restrict__ nhm, int n) {
@@ -179,7 +179,7 @@ __global__ void k_wl_items(const uint8_t* __restrict__ hs, const uint8_t* __rest
 
 // signatures sk[key_idx[i]] * H_i of given points H_i (reference form, 12 u64 each),
 // compressed (48 bytes)
-__global__ void k_wl_sign_points(const u64* __restrict__ h_ref, const uint8_t* __restrict__ sk,
+__global__ __launch_bounds__(64, 1) void k_wl_sign_points(const u64* __restrict__ h_ref, const uint8_t* __restrict__ sk,
                                  const int32_t* __restrict__ key_idx, uint8_t* __restrict__ sig48, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
