@@ -437,7 +437,7 @@ int blsgpu_register_key_coeffs(const uint64_t coeffs[2448]) {
   CHK(g.pk_ref.ensure(2448 * 8));
   HIPCHK(hipMemcpyAsync(g.pk_ref.p, coeffs, 2448 * 8, hipMemcpyHostToDevice, g.s));
   CHK(bls::launch_lines_from_ref(g.pk_ref.as<uint64_t>(),
-                                 g.keys.as<int32_t>() + (size_t)g.nkeys * bls::KEY_TABLE_WORDS, 1, 0, g.s));
+                                 g.keys.as<int32_t>() + (size_t)g.nkeys * bls::KEY_TABLE_WORDS, 1, g.s));
   HIPCHK(hipStreamSynchronize(g.s));
   return g.nkeys++;
 }

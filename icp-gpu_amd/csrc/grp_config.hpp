@@ -36,7 +36,7 @@ constexpr bool rides(int which) { return which == 0 ? (G == 8 || G == 12) : (G =
 // Per-G launchers and occupancy queries, instantiated in the translation unit of G.
 template <int G>
 int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                  const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                  const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
                   int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
 template <int G>
 int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,

@@ -16,11 +16,11 @@ int grp_num_cus() {
 }
 
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                      const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
                       int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   switch (G) {
 #define ML_CASE(g) \
-  case g: return grp_launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_soa, status, fout, M, queue, wg_per_cu, riders, s);
+  case g: return grp_launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_items, status, fout, M, queue, wg_per_cu, riders, s);
     BLS_GRP_ML_LANES(ML_CASE)
 #undef ML_CASE
     default: return -1;

@@ -201,52 +201,6 @@ __device__ __forceinline__ void hash_to_field2(const uint8_t* msg, int len, cons
   u0 = field_from_512(ub[0], ub[1]);
   u1 = field_from_512(ub[2], ub[3]);
 }
-// ---- the cofactor chain on a lane pair ----
-// Both lanes of a pair (lanes 2k, 2k + 1) hold the same point.  A doubling's seven field
-// products run as three steps of one product per lane plus one the pair computes twice,
-// each lane reading its partner's result by DPP (quad_perm 1,0,3,2: lane i reads i ^ 1).
-__device__ __forceinline__ Fp pair_swap(const Fp& a) {
-  Fp r;
-#pragma unroll
-  for (int j = 0; j < NL; j++) r.l[j] = __builtin_amdgcn_update_dpp(a.l[j], a.l[j], 0xB1, 0xF, 0xF, false);
-  return r;
-}
-__device__ __forceinline__ Fp pick(bool c, const Fp& a, const Fp& b) {
-  Fp r;
-#pragma unroll
-  for (int j = 0; j < NL; j++) r.l[j] = c ? a.l[j] : b.l[j];
-  return r;
-}
-// dbl-2009-l (the formulas of jdbl) with the products split over the pair; odd = the
-// lane's parity.  Step 1: A = X^2 | B = Y^2; step 2: C = B^2 | T = (X + B)^2;
-// step 3: Y Z | F = E^2; then E (D - X3) on both lanes.
-__device__ __forceinline__ Jac<Fp> jdbl_pair(const Jac<Fp>& p, bool odd) {
-  const Fp s1 = sqr(pick(odd, p.x, p.y)), s1o = pair_swap(s1);
-  const Fp A = pick(odd, s1, s1o), B = pick(odd, s1o, s1);
-  const Fp s2 = sqr(pick(odd, B, carry(add(p.x, B)))), s2o = pair_swap(s2);
-  const Fp C = pick(odd, s2, s2o), T = pick(odd, s2o, s2);
-  const Fp D = norm(dbl(sub(sub(T, A), C)));
-  const Fp E = norm(add(dbl(A), A));
-  const Fp s3 = mul(pick(odd, p.y, E), pick(odd, p.z, E)), s3o = pair_swap(s3);
-  const Fp YZ = pick(odd, s3, s3o), F = pick(odd, s3o, s3);
-  Jac<Fp> r;
-  r.x = norm(sub(F, dbl(D)));
-  const Fp C8 = norm(dbl(dbl(dbl(C))));
-  r.y = norm(sub(mul(E, sub(D, r.x)), C8));
-  r.z = norm(dbl(YZ));
-  return r;
-}
-// [|x|] q on a lane pair (as mul_absx_jac; the additions run on both lanes)
-__device__ __forceinline__ Jac<Fp> mul_absx_pair(const Jac<Fp>& q, bool odd) {
-  Jac<Fp> acc = q;
-#pragma unroll 1
-  for (int i = 62; i >= 0; i--) {
-    acc = jdbl_pair(acc, odd);
-    if ((BLS_X >> i) & 1) acc = jadd_jac<FieldInline>(acc, q);
-  }
-  return acc;
-}
-
 // [1 - x](Q0 + Q1) (cofactor clearing, sec. 7 with h_eff of sec. 8.8.1) as affine
 // (x, y) in internal form, on both lanes of a pair; false for the identity
 __device__ __forceinline__ bool clear_cofactor_affine(const Jac<Fp>& q0, const Jac<Fp>& q1, bool odd, Fp& x,

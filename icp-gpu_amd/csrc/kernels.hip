@@ -5,9 +5,8 @@
 //   G1 points    : SoA over items, word w (0..27 = x limbs, y limbs) of item i at
 //                  pts[w * M + i]  (coalesced across a wavefront)
 //   line tables  : one key = 68 lines x (c0, c1, c2 in Fp2) = 68 x 84 words; per-key
-//                  tables are stored key-major (AoS, read with 16-byte loads), per-item
-//                  tables of the legacy ABI are stored SoA: word w of line k of item i
-//                  at lines[(k * 84 + w) * M + i]
+//                  tables and the per-item tables of the legacy ABI are stored key-major
+//                  (one table after the other, read with 16-byte loads)
 //   Miller value : SoA, 168 words (12 Fp) per item
 #include <hip/hip_runtime.h>
 
@@ -62,18 +61,6 @@ struct LinesAoS {  // one key table (same for every lane or chosen per lane)
     return L;
   }
 };
-struct LinesSoA {  // per-item tables of the legacy ABI
-  const i32* t;
-  int M, i;
-  __device__ __forceinline__ Line operator()(int k) const {
-    Line L;
-    Fp* f = &L.c0.c0;
-#pragma unroll
-    for (int e = 0; e < 6; e++) f[e] = ld_fp_soa(t, k * LINE_WORDS + e * NL, M, i);
-    return L;
-  }
-};
-
 // ---- kernels ----
 
 // legacy ABI points: M x 12 u64 (x | y, reference Montgomery form) -> internal SoA
@@ -84,16 +71,16 @@ __global__ void k_points_from_ref(const u64* __restrict__ in, i32* __restrict__ 
   st_fp_soa(out, NL, M, i, ld_ref(in + (size_t)i * 12 + 6));
 }
 
-// legacy ABI line tables: M x 2448 u64 (68 x 6 Fp, reference form) -> internal.
-// One thread per (item, Fp); soa != 0 writes the per-item SoA layout, else key-major AoS.
-__global__ void k_lines_from_ref(const u64* __restrict__ in, i32* __restrict__ out, int M, int soa) {
+// legacy ABI line tables: M x 2448 u64 (68 x 6 Fp, reference form) -> internal tables,
+// one after the other (the key-major layout of the resident keys).  One thread per
+// (item, Fp), Fp fastest: a wavefront reads 64 x 48 contiguous bytes and writes 64 x 56.
+__global__ void k_lines_from_ref(const u64* __restrict__ in, i32* __restrict__ out, int M) {
   long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int per = NUM_COEFFS * 6;  // 408 Fp per key
   if (g >= (long long)M * per) return;
   int i = (int)(g / per), f = (int)(g % per);
   Fp v = ld_ref(in + (size_t)i * (per * 6) + (size_t)f * 6);
-  if (soa) st_fp_soa(out, f * NL, M, i, v);
-  else st_fp_aos(out + (size_t)i * KEY_WORDS + f * NL, v);
+  st_fp_aos(out + (size_t)i * KEY_WORDS + f * NL, v);
 }
 
 // G2Prepared of K affine points given in reference form (24 u64: x.c0 x.c1 y.c0 y.c1)
@@ -170,8 +157,8 @@ __global__ void k_points_to_ref(const i32* __restrict__ pts, const i32* __restri
 }
 
 // two-pair Miller loop.  Pair 1: (sig, G2 generator lines); pair 2: (-H(m), pk lines).
-// pk lines come from per-item SoA tables (key_idx == nullptr) or from key-major tables
-// selected by key_idx[i].  sig_status (nullable): DEC_IDENTITY drops pair 1, any other
+// pk lines come from the item's own table (key_idx == nullptr: the legacy ABI's tables,
+// one per item) or from the key table selected by key_idx[i].  sig_status (nullable): DEC_IDENTITY drops pair 1, any other
 // non-OK status skips the item (its verdict is set by the final kernel).
 __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nhm, const i32* __restrict__ g2lines,
                          const i32* __restrict__ pklines, const int32_t* __restrict__ key_idx, int nkeys,
@@ -192,13 +179,9 @@ __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nh
   G1Aff p2 = {ld_fp_soa(nhm, 0, M, i), ld_fp_soa(nhm, NL, M, i)};
   Fp12 f;
   LinesAoS gen{g2lines};
-  if (key_idx) {
-    LinesAoS pk{pklines + (size_t)key_idx[i] * KEY_WORDS};
-    f = miller2(p1, gen, p2, pk, skip1, false);
-  } else {
-    LinesSoA pk{pklines, M, i};
-    f = miller2(p1, gen, p2, pk, skip1, false);
-  }
+  // the item's key table, or its own table (legacy ABI)
+  const LinesAoS pk{pklines + (size_t)(key_idx ? key_idx[i] : i) * KEY_WORDS};
+  f = miller2(p1, gen, p2, pk, skip1, false);
   const Fp* v = &f.c0.c0.c0;
 #pragma unroll
   for (int e = 0; e < 12; e++) st_fp_soa(fout, e * NL, M, i, v[e]);
@@ -248,9 +231,9 @@ int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t 
   hipLaunchKernelGGL(k_points_from_ref, dim3(nblk(M, 64)), dim3(64), 0, s, in, out, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
-int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, int soa, hipStream_t s) {
+int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s) {
   long long n = (long long)M * NUM_COEFFS * 6;
-  hipLaunchKernelGGL(k_lines_from_ref, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, M, soa);
+  hipLaunchKernelGGL(k_lines_from_ref, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s) {

@@ -11,7 +11,8 @@ constexpr int POINT_WORDS = 28;           // internal affine G1 point
 constexpr int MILLER_WORDS = 168;         // internal Fp12
 
 int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s);
-int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, int soa, hipStream_t s);
+// M tables of 2448 u64 (reference form) -> M internal tables of KEY_TABLE_WORDS, one after the other
+int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s);
 int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s);
 int launch_g2_prepare(const uint64_t* q, int32_t* out, int K, hipStream_t s);
 // n messages (bytes msgs[offs[i] .. offs[i+1])) -> H(m) (RFC 9380 hash to G1 with the
@@ -49,7 +50,7 @@ struct SubgroupRiders {
 // workgroups per CU, 0 = the occupancy limit) pops tiles from the device counter *queue,
 // which must be zero at launch; riders must then be empty.
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                      const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                      const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
                       int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
                          uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);

@@ -1,7 +1,8 @@
 // G1 signature decoding, split in two kernels so that the subgroup test can run
 // concurrently with the Miller loop (which needs only the decoded coordinates):
 //   k_g1_decode   flags, x < p, y = sqrt(x^3 + 4) with the sign choice -> point + status
-//   k_g1_subgroup [x^2] P == (beta x, -y) on decoded points -> subgroup status
+//   k_g1_subgroup [x^2] P == (beta x, -y) on decoded points -> subgroup status (two
+//                 lanes per item)
 // Semantics as ic_bls12_381 G1Affine::from_compressed; the reference GPU decoder
 // (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the decoding steps
 // without any of the validity checks.  Compiled with the field multiplier inlined.
@@ -33,12 +34,30 @@ __global__ __launch_bounds__(64, 1) void k_g1_decode(const uint8_t* __restrict__
   status[i] = st;
 }
 
-// one thread per item (g1_subgroup_status)
+// The test of g1_subgroup_status on a lane pair: lanes 2k and 2k + 1 of the grid hold
+// item k and split the doublings of [|x|]([|x|]P) (jdbl_pair: four product latencies per
+// doubling instead of seven); the additions and the comparison run on both lanes, and
+// the even lane writes the status.  Both lanes of a pair take every branch together
+// (they hold the same values), so the partner exchange always finds its partner active.
 __global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
                                                     i32* __restrict__ sub_status, int M) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t >> 1;
+  const bool odd = t & 1;
   if (i >= M) return;
-  sub_status[i] = g1_subgroup_status(pts, status, M, i);
+  if (status[i] != DEC_OK) {
+    if (!odd) sub_status[i] = DEC_OK;
+    return;
+  }
+  const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
+  const Jac<Fp> p1 = {x, y, fp_one()};
+  const Jac<Fp> b = mul_absx_pair(mul_absx_pair(p1, odd), odd);
+  bool in = false;
+  if (!is_zero(b.z)) {
+    const Fp z2 = sqr(b.z);
+    in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
+  }
+  if (!odd) sub_status[i] = in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }
 
 // status := sub_status where status is OK (for entry points that report decoding alone)
@@ -80,7 +99,7 @@ int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hi
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s) {
-  hipLaunchKernelGGL(k_g1_subgroup, dim3((M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
+  hipLaunchKernelGGL(k_g1_subgroup, dim3((2 * (size_t)M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_merge_status(int32_t* status, const int32_t* sub_status, int M, hipStream_t s) {

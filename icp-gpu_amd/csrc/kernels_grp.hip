@@ -426,7 +426,7 @@ template <int G, int WPE>
 __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                             const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                             const int32_t* __restrict__ key_idx, int nkeys,
-                                            const i32* __restrict__ pk_soa, i32* __restrict__ status,
+                                            const i32* __restrict__ pk_items, i32* __restrict__ status,
                                             i32* __restrict__ fout, int M) {
   constexpr int IPW = Cfg<G>::IPW, L = Cfg<G>::L;
   const int ln = threadIdx.x & 63, lid = ln / L, sub = (int)(threadIdx.x >> 6) * L + ln % L;
@@ -470,7 +470,9 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
   }
   __syncthreads();
   if (in_wg && sub == 0) lds_st(F, fp_one());
-  const i32* pk = key_idx ? keys + (size_t)kidx * KEY_TABLE_WORDS : nullptr;
+  // this item's line table: its key's resident table, or its own (legacy ABI: the tables
+  // of the M items one after the other)
+  const i32* pk = key_idx ? keys + (size_t)kidx * KEY_TABLE_WORDS : pk_items + (size_t)it * KEY_TABLE_WORDS;
   __syncthreads();
 
   // lines of step ci into L1 | L2: generator (same for every item) and this item's key;
@@ -481,7 +483,7 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
         const int which = w / 84, ww = w % 84, e = ww / NL, l = ww % NL;
         i32 v;
         if (which == 0) v = gen_lines[ci * 84 + ww];
-        else v = pk ? pk[ci * 84 + ww] : pk_soa[(size_t)(ci * 84 + ww) * M + it];
+        else v = pk[ci * 84 + ww];
         g_lds[(which == 0 ? L1 : L2) + e * SLOT + l] = v;
       }
     }
@@ -498,7 +500,7 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
 #pragma unroll
       for (int k = 0; k < LW; k++) {
         const int w = sub + k * G;
-        if (w < 84) v[k] = pk ? pk[ci * 84 + w] : pk_soa[(size_t)(ci * 84 + w) * M + it];
+        if (w < 84) v[k] = pk[ci * 84 + w];
       }
     }
   };
@@ -615,7 +617,7 @@ template <int G, bool RIDE>
 __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                                    const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                                    const int32_t* __restrict__ key_idx, int nkeys,
-                                                   const i32* __restrict__ pk_soa, i32* __restrict__ status,
+                                                   const i32* __restrict__ pk_items, i32* __restrict__ status,
                                                    i32* __restrict__ fout, int M, int* queue, SubgroupRiders riders) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
@@ -633,7 +635,7 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_miller_grp(co
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
     if (tile >= ntiles) break;
-    miller_tile<G, RIDE ? 1 : 2>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, pk_soa, status, fout, M);
+    miller_tile<G, RIDE ? 1 : 2>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, pk_items, status, fout, M);
   }
 }
 
@@ -845,7 +847,7 @@ static int launch_grid(int occ, int ntiles, int threads, int* queue, int wg_per_
 
 template <int G>
 int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                  const int32_t* key_idx, int nkeys, const int32_t* pk_soa, int32_t* status, int32_t* fout, int M,
+                  const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
                   int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
@@ -861,12 +863,12 @@ int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, co
   if constexpr (rides<G>(0)) {
     if (riders.per || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W)) {
       hipLaunchKernelGGL((k_miller_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen,
-                         keys, key_idx, nkeys, pk_soa, status, fout, M, queue, riders);
+                         keys, key_idx, nkeys, pk_items, status, fout, M, queue, riders);
       return hipGetLastError() == hipSuccess ? 0 : -1;
     }
   }
   hipLaunchKernelGGL((k_miller_grp<G, false>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen, keys,
-                     key_idx, nkeys, pk_soa, status, fout, M, queue, riders);
+                     key_idx, nkeys, pk_items, status, fout, M, queue, riders);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int G>

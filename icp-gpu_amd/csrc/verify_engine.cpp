@@ -6,11 +6,11 @@
 namespace bls {
 
 static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, const int32_t* key_idx,
-                  const int32_t* pk_soa, int32_t* status, hipStream_t s, const SubgroupRiders& riders = {}) {
+                  const int32_t* pk_items, int32_t* status, hipStream_t s, const SubgroupRiders& riders = {}) {
   if (a.mode == 0)
-    return launch_miller(a.sig_pts, a.nhm_pts, gen, key_idx ? keys : pk_soa, key_idx, a.nkeys, status, a.fbuf, a.M,
+    return launch_miller(a.sig_pts, a.nhm_pts, gen, key_idx ? keys : pk_items, key_idx, a.nkeys, status, a.fbuf, a.M,
                          a.block, s);
-  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_soa, status, a.fbuf, a.M,
+  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_items, status, a.fbuf, a.M,
                            a.queue && a.wpc_ml >= 0 ? a.queue : nullptr, a.wpc_ml, riders, s);
 }
 static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* sub_status, hipStream_t s,
@@ -26,8 +26,8 @@ static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* 
 int engine_legacy(const EngineArgs& a, hipStream_t s) {
   if (launch_points_from_ref(a.sig_ref, a.sig_pts, a.M, s)) return -1;
   if (launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
-  if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, 0, s)) return -1;
-  if (launch_lines_from_ref(a.pk_ref, a.pk_lines, a.M, 1, s)) return -1;
+  if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, s)) return -1;
+  if (launch_lines_from_ref(a.pk_ref, a.pk_lines, a.M, s)) return -1;
   if (miller(a, a.g2_lines, nullptr, nullptr, a.pk_lines, nullptr, s)) return -1;
   return final_exp(a, nullptr, nullptr, s);
 }

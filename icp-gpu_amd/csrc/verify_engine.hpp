@@ -22,7 +22,7 @@ struct EngineArgs {
   int32_t* sig_pts = nullptr;   // M x 28
   int32_t* nhm_pts = nullptr;   // M x 28
   int32_t* g2_lines = nullptr;  // 5712 (legacy)
-  int32_t* pk_lines = nullptr;  // M x 5712 (legacy)
+  int32_t* pk_lines = nullptr;  // legacy: the M items' internal line tables, 5712 words each
   int32_t* status = nullptr;    // M (fused): decode status
   int32_t* sub_status = nullptr; // M (fused): subgroup status
   int32_t* fbuf = nullptr;      // M x 168 Miller values: word-major (SoA) in mode 0, item-major in mode 1
