@@ -51,6 +51,8 @@ EXTENSION_SYMBOLS = (
     "blsgpu_set_batched_engine",
     "blsgpu_get_batched_engine",
     "blsgpu_batched_stats",
+    "blsgpu_set_line_depth",
+    "blsgpu_get_line_depth",
     "blsgpu_autotune_device",
     "blsgpu_server_create",
     "blsgpu_server_step_capacity",
@@ -119,6 +121,8 @@ def load(path=None):
         "blsgpu_set_batched_engine": ([i32, i32, i32, i32, i32], i32),
         "blsgpu_get_batched_engine": ([i32, vp], i32),
         "blsgpu_batched_stats": ([vp], i32),
+        "blsgpu_set_line_depth": ([i32], i32),
+        "blsgpu_get_line_depth": ([i32, i32], i32),
         "blsgpu_autotune_device": ([vp, vp, vp, i32, i32, vp, vp, vp], i32),
         "blsgpu_server_create": ([i32, i32, i64, i32, i32], vp),
         "blsgpu_server_step_capacity": ([vp], i32),
@@ -315,6 +319,16 @@ def get_batched_engine(m=1):
     cfg = np.zeros(5, dtype=np.int32)
     load().blsgpu_get_batched_engine(m, _p(cfg))
     return tuple(int(v) for v in cfg)
+
+
+def set_line_depth(depth):
+    """force the Miller loop's key-line depth (0, 1, 2) on every form; -1 = autotuned/default"""
+    _check(load().blsgpu_set_line_depth(int(depth)), "blsgpu_set_line_depth")
+
+
+def get_line_depth(form, m):
+    """key-line depth a call of the plain form (form=1) or batched form (form=0) of m items runs"""
+    return load().blsgpu_get_line_depth(int(form), int(m))
 
 
 def batched_stats():

@@ -174,6 +174,7 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   CHK(g.nhm_pts.ensure(m * bls::POINT_WORDS * 4));
   CHK(g.g2_lines.ensure(bls::KEY_TABLE_WORDS * 4));
   CHK(g.pk_lines.ensure(m * bls::KEY_TABLE_WORDS * 4));
+  CHK(g.keyidx.ensure(m * 4));
   CHK(g.res.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
@@ -203,6 +204,7 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   a.mode = g.mode;
   a.g_ml = g.g_ml;
   a.g_fe = g.g_fe;
+  a.depth_ml = plain_depth_locked();
   a.sig_ref = g.sig_ref.as<uint64_t>();
   a.nhm_ref = g.nhm_ref.as<uint64_t>();
   a.g2_ref = g.g2_ref.as<uint64_t>();
@@ -211,6 +213,7 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   a.nhm_pts = g.nhm_pts.as<int32_t>();
   a.g2_lines = g.g2_lines.as<int32_t>();
   a.pk_lines = g.pk_lines.as<int32_t>();
+  a.item_idx = g.keyidx.as<int32_t>();
   a.fbuf = g.fbuf.as<int32_t>();
   a.results = g.res.as<int32_t>();
   a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
@@ -577,6 +580,7 @@ static int verify_compressed_locked(const uint8_t* sigs48, const uint64_t* neg_h
   a.mode = g.mode;
   a.g_ml = g.g_ml;
   a.g_fe = g.g_fe;
+  a.depth_ml = plain_depth_locked();
   a.comp = g.comp.as<uint8_t>();
   a.nhm_ref = g.nhm_ref.as<uint64_t>();
   a.key_idx = g.keyidx.as<int32_t>();
@@ -645,6 +649,7 @@ int blsgpu_verify_compressed_device(const uint8_t* d_sigs48, const uint64_t* d_n
   a.mode = g.mode;
   a.g_ml = g.g_ml;
   a.g_fe = g.g_fe;
+  a.depth_ml = plain_depth_locked();
   a.comp = d_sigs48;
   a.nhm_ref = d_neg_hms;
   a.key_idx = d_key_idx;
@@ -708,6 +713,7 @@ int blsgpu_verify_compressed_device_noalloc(const uint8_t* d_sigs48, const uint6
   a.mode = g.mode;
   a.g_ml = g.g_ml;
   a.g_fe = g.g_fe;
+  a.depth_ml = plain_depth_locked();
   a.comp = d_sigs48;
   a.nhm_ref = d_neg_hms;
   a.key_idx = d_key_idx;
@@ -742,6 +748,7 @@ int blsgpu_set_engine(int mode, int g_miller, int g_final_exp) {
   if (mode == 1) {
     g.g_ml = g_miller;
     g.g_fe = g_final_exp;
+    g.depth_ml = -1;  // a tuned depth belongs to the engine it was tuned with
   }
   return 0;
 }

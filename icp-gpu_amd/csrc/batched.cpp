@@ -35,7 +35,7 @@ bool GraphKey::operator==(const GraphKey& o) const {
   for (int i = 0; i < 7; i++)
     if (ws[i] != o.ws[i]) return false;
   return keys == o.keys && nkeys == o.nkeys && M == o.M && cfg.g_ml == o.cfg.g_ml && cfg.g_fe == o.cfg.g_fe &&
-         cfg.wpc_ml == o.cfg.wpc_ml && cfg.wpc_fe == o.cfg.wpc_fe;
+         cfg.wpc_ml == o.cfg.wpc_ml && cfg.wpc_fe == o.cfg.wpc_fe && cfg.depth_ml == o.cfg.depth_ml;
 }
 
 namespace {
@@ -52,12 +52,16 @@ int size_class(int M) {
 }  // namespace
 
 BatchedCfg cfg_for_locked(int M) {
-  if (g.tuned.empty()) return g.bcfg;
-  const int c = size_class(M);
-  auto best = g.tuned.begin();
-  for (auto it = g.tuned.begin(); it != g.tuned.end(); ++it)
-    if (std::abs(it->first - c) < std::abs(best->first - c)) best = it;
-  return best->second;
+  BatchedCfg r = g.bcfg;
+  if (!g.tuned.empty()) {
+    const int c = size_class(M);
+    auto best = g.tuned.begin();
+    for (auto it = g.tuned.begin(); it != g.tuned.end(); ++it)
+      if (std::abs(it->first - c) < std::abs(best->first - c)) best = it;
+    r = best->second;
+  }
+  if (g.force_depth >= 0) r.depth_ml = g.force_depth;
+  return r;
 }
 
 namespace {
@@ -85,6 +89,7 @@ bls::EngineArgs batched_args(const uint8_t* comp, const uint64_t* nhm, const int
   a.g_fe = c.g_fe;
   a.wpc_ml = c.wpc_ml;
   a.wpc_fe = c.wpc_fe;
+  a.depth_ml = c.depth_ml;
   a.queue = g.b_queue.as<int>();
   a.comp = comp;
   a.nhm_ref = nhm;
@@ -302,6 +307,43 @@ int blsgpu_get_batched_engine(int M, int cfg[5]) {
   return 0;
 }
 
+int blsgpu_set_line_depth(int depth) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if (depth < -1 || depth > 2) {
+    g_err = "key-line depth must be -1, 0, 1 or 2";
+    return -1;
+  }
+  g.force_depth = depth;
+  return 0;
+}
+
+int blsgpu_get_line_depth(int form, int M) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  g_err.clear();
+  if ((form != 0 && form != 1) || M <= 0) {
+    g_err = "invalid arguments";
+    return -1;
+  }
+  if (form == 1 && g.mode != 1) return 0;
+  bls::EngineArgs a{};
+  a.M = M;
+  a.mode = 1;
+  int dummy_queue = 0;
+  if (form == 1) {
+    a.g_ml = g.g_ml;
+    a.depth_ml = plain_depth_locked();
+  } else {
+    const BatchedCfg c = cfg_for_locked(M);
+    a.g_ml = c.g_ml;
+    a.wpc_ml = c.wpc_ml;
+    a.depth_ml = c.depth_ml;
+    a.queue = &dummy_queue;
+  }
+  const bool tiles = !(a.queue && a.wpc_ml >= 0);
+  return bls::engine_miller_depth(a, tiles && bls::grp_rider_items(0, a.g_ml, M) > 0);
+}
+
 int blsgpu_batched_stats(int out[2]) {
   std::lock_guard<std::mutex> lk(g.mu);
   if (!out) return -1;
@@ -335,6 +377,7 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
     }
   } guard{e0, e1};
   BatchedCfg c = g.bcfg;
+  c.depth_ml = g.force_depth;
   bls::EngineArgs a =
       batched_args(d_sigs48, d_neg_hms, d_key_idx, g.b_res.as<int32_t>(), nullptr, M, c);
   CHK(bls::engine_prepare(a, g.s));
@@ -413,12 +456,37 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
       }
     }
   }
+  // 3. the key-line depth of the chosen Miller configuration (unless one is forced):
+  // every depth its instantiation holds, in the whole sequence
+  if (g.force_depth < 0) {
+    a.g_ml = c.g_ml;
+    a.wpc_ml = c.wpc_ml;
+    a.g_fe = c.g_fe;
+    a.wpc_fe = c.wpc_fe;
+    const bool tiles = !(a.queue && a.wpc_ml >= 0);
+    const bool riders = tiles && bls::grp_rider_items(0, a.g_ml, M) > 0;
+    for (int d = 0; d <= 2; d++) {
+      a.depth_ml = d;
+      if (bls::engine_miller_depth(a, riders) != d) continue;  // runs as a smaller depth
+      double t = 0;
+      CHK(time_phase(2, &t, best_ml));
+      if (t < best_ml) {
+        best_ml = t;
+        c.depth_ml = d;
+      }
+    }
+    if (c.depth_ml < 0) {
+      a.depth_ml = -1;
+      c.depth_ml = bls::engine_miller_depth(a, riders);
+    }
+  }
   if (form == 0) {
     g.tuned[size_class(M)] = c;
   } else {
     g.mode = 1;
     g.g_ml = c.g_ml;
     g.g_fe = c.g_fe;
+    g.depth_ml = c.depth_ml;
   }
   if (cfg_out) {
     cfg_out[0] = c.g_ml;

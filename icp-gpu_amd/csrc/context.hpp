@@ -77,6 +77,7 @@ struct PinnedBuf {
 // workgroups per CU (-1 = one workgroup per tile without a queue, 0 = occupancy limit).
 struct BatchedCfg {
   int g_ml = 12, g_fe = 16, wpc_ml = 0, wpc_fe = 0;
+  int depth_ml = -1;  // key-line depth of the Miller loop (-1 = the default of g_ml)
 };
 
 // One instantiated graph of the batched form and everything its nodes captured.
@@ -125,8 +126,10 @@ struct Ctx {
   // the legacy entry point's host-to-device copies, kernels and device-to-host copies
   double legacy_ms[3] = {0, 0, 0};
   int legacy_calls = 0;
-  // engine configuration of the plain form (blsgpu_set_engine)
+  // engine configuration of the plain form (blsgpu_set_engine), its Miller key-line depth
+  // (autotuned, -1 = the default) and a depth forced on every form (blsgpu_set_line_depth)
   int mode = 1, g_ml = 12, g_fe = 16;
+  int depth_ml = -1, force_depth = -1;
   // batched (queue-fed) form: its own workspace, the configuration by batch-size class
   // (ceil(log2 M) -> autotuned configuration) and the graph of the last call
   DevBuf b_comp, b_nhm, b_keyidx, b_res, b_gt;
@@ -144,7 +147,10 @@ extern Ctx g;
 int init_locked();
 // wait for the stream that last used the shared workspace if it differs from s
 int claim_workspace_locked(hipStream_t s);
-// configuration of the batched form for a job of M items (autotuned class or default)
+// configuration of the batched form for a job of M items (autotuned class or default),
+// with the forced key-line depth applied
 BatchedCfg cfg_for_locked(int M);
+// key-line depth of the plain form's Miller loop (forced, else autotuned, else -1)
+inline int plain_depth_locked() { return g.force_depth >= 0 ? g.force_depth : g.depth_ml; }
 
 }  // namespace blsctx

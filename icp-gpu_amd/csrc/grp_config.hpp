@@ -33,11 +33,33 @@ struct Cfg {
 template <int G>
 constexpr bool rides(int which) { return which == 0 ? (G == 8 || G == 12) : (G == 9 || G == 12); }
 
+// Key-line depth of the Miller loop: how many steps ahead of its use each step's line of
+// the item's key is read from memory into registers (0: read at the step that stores it
+// to LDS; 1: one step ahead; 2: two steps ahead).  A depth is available where the held
+// words (LW = ceil(84 / G) per lane and step) fit the kernel's register budget: LW <= 4
+// per step held with the two-wave budget, LW <= 7 with the one-wave budget.
+template <int G>
+constexpr bool ml_depth_ok(int D, bool one_wave) {
+  constexpr int LW = (84 + G - 1) / G;
+  return D == 0 || (D >= 1 && D <= 2 && D * LW <= (one_wave ? 14 : 8) && LW <= (one_wave ? 7 : 4));
+}
+// the depth a launch runs: the requested one if available, else the largest available
+// below it; requested < 0 = the default (1 where available, else 0)
+template <int G>
+constexpr int ml_depth_for(bool one_wave, int requested) {
+  int d = requested < 0 ? 1 : requested > 2 ? 2 : requested;
+  while (d > 0 && !ml_depth_ok<G>(d, one_wave)) d--;
+  return d;
+}
+
 // Per-G launchers and occupancy queries, instantiated in the translation unit of G.
 template <int G>
 int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                  const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
-                  int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+                  const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout, int M,
+                  int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s);
+// the depth grp_launch_ml runs for these arguments
+template <int G>
+int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth);
 template <int G>
 int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
                   int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);

@@ -16,11 +16,21 @@ int grp_num_cus() {
 }
 
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                      const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
-                      int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
+                      const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout, int M,
+                      int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s) {
   switch (G) {
-#define ML_CASE(g) \
-  case g: return grp_launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, pk_items, status, fout, M, queue, wg_per_cu, riders, s);
+#define ML_CASE(g)                                                                                                  \
+  case g:                                                                                                           \
+    return grp_launch_ml<g>(sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue, wg_per_cu, depth,        \
+                            riders, s);
+    BLS_GRP_ML_LANES(ML_CASE)
+#undef ML_CASE
+    default: return -1;
+  }
+}
+int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth) {
+  switch (G) {
+#define ML_CASE(g) case g: return grp_ml_depth<g>(queue, wg_per_cu, riders, depth);
     BLS_GRP_ML_LANES(ML_CASE)
 #undef ML_CASE
     default: return -1;

@@ -157,8 +157,8 @@ __global__ void k_points_to_ref(const i32* __restrict__ pts, const i32* __restri
 }
 
 // two-pair Miller loop.  Pair 1: (sig, G2 generator lines); pair 2: (-H(m), pk lines).
-// pk lines come from the item's own table (key_idx == nullptr: the legacy ABI's tables,
-// one per item) or from the key table selected by key_idx[i].  sig_status (nullable): DEC_IDENTITY drops pair 1, any other
+// pk lines come from the key table selected by key_idx[i] (the legacy ABI's per-item
+// tables come as M keys with key_idx[i] = i).  sig_status (nullable): DEC_IDENTITY drops pair 1, any other
 // non-OK status skips the item (its verdict is set by the final kernel).
 __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nhm, const i32* __restrict__ g2lines,
                          const i32* __restrict__ pklines, const int32_t* __restrict__ key_idx, int nkeys,
@@ -171,7 +171,7 @@ __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nh
     if (st != DEC_OK && st != DEC_IDENTITY) return;
     skip1 = st == DEC_IDENTITY;
   }
-  if (key_idx && (key_idx[i] < 0 || key_idx[i] >= nkeys)) {
+  if (key_idx[i] < 0 || key_idx[i] >= nkeys) {
     if (sig_status) sig_status[i] = DEC_BAD_KEY;
     return;
   }
@@ -179,8 +179,7 @@ __global__ void k_miller(const i32* __restrict__ sig, const i32* __restrict__ nh
   G1Aff p2 = {ld_fp_soa(nhm, 0, M, i), ld_fp_soa(nhm, NL, M, i)};
   Fp12 f;
   LinesAoS gen{g2lines};
-  // the item's key table, or its own table (legacy ABI)
-  const LinesAoS pk{pklines + (size_t)(key_idx ? key_idx[i] : i) * KEY_WORDS};
+  const LinesAoS pk{pklines + (size_t)key_idx[i] * KEY_WORDS};
   f = miller2(p1, gen, p2, pk, skip1, false);
   const Fp* v = &f.c0.c0.c0;
 #pragma unroll

@@ -49,9 +49,13 @@ struct SubgroupRiders {
 // workgroups of subgroup tests.  Otherwise a persistent grid (at most wg_per_cu
 // workgroups per CU, 0 = the occupancy limit) pops tiles from the device counter *queue,
 // which must be zero at launch; riders must then be empty.
+// depth: the Miller loop's key-line depth (steps ahead its key lines are read into
+// registers: 0, 1, 2; -1 = the default of G); a depth the instantiation's register budget
+// does not hold runs as the largest one below it (miller_grp_depth gives the one it runs).
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                      const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
-                      int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+                      const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout, int M,
+                      int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s);
+int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
                          uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
 // Items per subgroup-test workgroup that a one-workgroup-per-tile launch of M items at G
@@ -66,6 +70,8 @@ bool grp_fills_simds(int G, int M);
 int occupancy_grp(int which, int G);
 // p[0..n) = 0 by a kernel (safe inside captured graphs)
 int launch_zero_words(int* p, int n, hipStream_t s);
+// p[i] = i for i < n
+int launch_iota(int32_t* p, int n, hipStream_t s);
 
 // G1 decoding in two parts (kernels_g1.hip): decode (flags, range, square root, sign)
 // and the subgroup test, which may run concurrently with the Miller loop.

@@ -94,6 +94,14 @@ int launch_zero_words(int* p, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_words, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+__global__ void k_iota(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+int launch_iota(int32_t* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s) {
   hipLaunchKernelGGL(k_g1_decode, dim3((M + 63) / 64), dim3(64), 0, s, in, pts, status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;

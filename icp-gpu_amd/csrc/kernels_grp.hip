@@ -422,11 +422,11 @@ constexpr size_t miller_lds_bytes() {
   return (size_t)(Cfg<G>::IPW * ML_SLOTS + sched::NUM_CONSTS) * SLOT * 4;
 }
 
-template <int G, int WPE>
+template <int G, int WPE, int D>
 __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                             const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                             const int32_t* __restrict__ key_idx, int nkeys,
-                                            const i32* __restrict__ pk_items, i32* __restrict__ status,
+                                            i32* __restrict__ status,
                                             i32* __restrict__ fout, int M) {
   constexpr int IPW = Cfg<G>::IPW, L = Cfg<G>::L;
   const int ln = threadIdx.x & 63, lid = ln / L, sub = (int)(threadIdx.x >> 6) * L + ln % L;
@@ -470,9 +470,10 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
   }
   __syncthreads();
   if (in_wg && sub == 0) lds_st(F, fp_one());
-  // this item's line table: its key's resident table, or its own (legacy ABI: the tables
-  // of the M items one after the other)
-  const i32* pk = key_idx ? keys + (size_t)kidx * KEY_TABLE_WORDS : pk_items + (size_t)it * KEY_TABLE_WORDS;
+  // this item's key table (the legacy ABI's per-item tables come as M keys with
+  // key_idx[i] = i), addressed as a 32-bit word offset from the uniform base: one
+  // register per lane instead of a 64-bit pointer (the tables stay below 2^31 words)
+  const u32 koff = (u32)kidx * (u32)KEY_TABLE_WORDS;
   __syncthreads();
 
   // lines of step ci into L1 | L2: generator (same for every item) and this item's key;
@@ -483,24 +484,25 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
         const int which = w / 84, ww = w % 84, e = ww / NL, l = ww % NL;
         i32 v;
         if (which == 0) v = gen_lines[ci * 84 + ww];
-        else v = pk[ci * 84 + ww];
+        else v = keys[koff + (u32)(ci * 84 + ww)];
         g_lds[(which == 0 ? L1 : L2) + e * SLOT + l] = v;
       }
     }
   };
   // the same in two halves for the key's line (read from HBM: the generator's lines are
-  // shared by every item and stay in L2), so that it is read from memory one step before
-  // it goes to LDS: lane sub holds words sub + k G (k < LW) of the 84.  Only where the
-  // held words fit the register budget (LW <= 4, or the one-wave budget); otherwise
-  // store_lines loads the step itself.
+  // shared by every item and stay in L2), so that it is read from memory D steps before
+  // it goes to LDS: lane sub holds words sub + k G (k < LW) of the 84 of each step held
+  // (lv the next one to store, lv2 the one after at depth 2).  At depth 0 store_lines
+  // loads the step itself.
   constexpr int LW = (84 + G - 1) / G;
-  constexpr bool PREFETCH = LW <= 4 || (WPE == 1 && LW <= 7);
+  static_assert(ml_depth_ok<G>(D, WPE == 1), "key-line depth beyond the register budget");
+  constexpr bool PREFETCH = D >= 1;
   auto fetch_lines = [&](int ci, i32(&v)[LW]) {
     if (PREFETCH && act) {
 #pragma unroll
       for (int k = 0; k < LW; k++) {
         const int w = sub + k * G;
-        if (w < 84) v[k] = pk[ci * 84 + w];
+        if (w < 84) v[k] = keys[koff + (u32)(ci * 84 + w)];
       }
     }
   };
@@ -522,17 +524,27 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
     // its two lines (OP_LL, then OP_FL_SCALE), and the scaling of the next step's lines
     // rides along with f * LL.  Region B of both ops is the line block L1 | L2 | P | S.
     constexpr int STEPS = KEY_TABLE_WORDS / 84;
-    i32 lv[LW];
+    i32 lv[LW], lv2[D == 2 ? LW : 1];
+    // step `next` to LDS, and the key lines held in registers advanced by one step
+    auto advance = [&](int next) {
+      store_lines(next, lv);
+      if constexpr (D == 1) {
+        if (next + 1 < STEPS) fetch_lines(next + 1, lv);
+      } else if constexpr (D == 2) {
+#pragma unroll
+        for (int k = 0; k < LW; k++) lv[k] = lv2[k];
+        if (next + 2 < STEPS) fetch_lines(next + 2, lv2);
+      }
+    };
     fetch_lines(0, lv);
-    store_lines(0, lv);
-    fetch_lines(1, lv);
+    if constexpr (D == 2) fetch_lines(1, lv2);
+    advance(0);
     __syncthreads();
     run_op<G, WPE>(sched::OP_SCALE2, sub, act, L1, P, S1, T, K);
     // L1 | L2 are free once S holds their scaled form: the raw lines of the next step
     // go to LDS before an op that ends in a barrier, ahead of the op that scales them,
-    // and the lines of the step after are read from memory meanwhile
-    store_lines(1, lv);
-    fetch_lines(2, lv);
+    // and the lines of the steps after are read from memory meanwhile
+    advance(1);
     run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
     // For b = 61 .. 0: a line step, a second one if bit b of |x| is set, the square of f;
     // then the last line step.  A line step multiplies f by LL, the product of its two
@@ -551,10 +563,7 @@ __device__ __forceinline__ void miller_tile(int tile, const i32* __restrict__ si
       }
       run_op_c<G, sched::OP_FL_SCALE>(fl0, sub, act, F, L1, F, T, K);
       ci++;
-      if (ci + 1 < STEPS) {
-        store_lines(ci + 1, lv);
-        if (ci + 2 < STEPS) fetch_lines(ci + 2, lv);
-      }
+      if (ci + 1 < STEPS) advance(ci + 1);
       if (!second && miller_bit(b)) {
         second = true;
         run_op_c<G, sched::OP_LL>(sub, act, 0, L1, S1, T, K);
@@ -613,11 +622,11 @@ __device__ __forceinline__ void subgroup_rider(const i32* pts, const i32* status
   if (t < per && i < M) out[i] = g1_subgroup_status(pts, status, M, i);
 }
 
-template <int G, bool RIDE>
+template <int G, bool RIDE, int D>
 __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_miller_grp(const i32* __restrict__ sig, const i32* __restrict__ nhm,
                                                    const i32* __restrict__ gen_lines, const i32* __restrict__ keys,
                                                    const int32_t* __restrict__ key_idx, int nkeys,
-                                                   const i32* __restrict__ pk_items, i32* __restrict__ status,
+                                                   i32* __restrict__ status,
                                                    i32* __restrict__ fout, int M, int* queue, SubgroupRiders riders) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
@@ -635,7 +644,7 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_miller_grp(co
   for (int round = 0;; round++) {
     const int tile = next_tile(queue, round);
     if (tile >= ntiles) break;
-    miller_tile<G, RIDE ? 1 : 2>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, pk_items, status, fout, M);
+    miller_tile<G, RIDE ? 1 : 2, D>(tile, sig, nhm, gen_lines, keys, key_idx, nkeys, status, fout, M);
   }
 }
 
@@ -805,7 +814,7 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_final_exp_grp
 template <int G>
 int grp_occ_ml() {
   static int occ = 0;
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G, false>, Cfg<G>::THREADS, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_miller_grp<G, false, ml_depth_for<G>(false, -1)>, Cfg<G>::THREADS, miller_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
 }
 template <int G>
@@ -845,10 +854,53 @@ static int launch_grid(int occ, int ntiles, int threads, int* queue, int wg_per_
   return 0;
 }
 
+// one instantiation of the Miller kernel, at the depth the launch resolved
+template <int G, bool RIDE, int D>
+static int launch_ml_kernel(int grid, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
+                            const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout,
+                            int M, int* queue, const SubgroupRiders& riders, hipStream_t s) {
+  hipLaunchKernelGGL((k_miller_grp<G, RIDE, D>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen,
+                     keys, key_idx, nkeys, status, fout, M, queue, riders);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template <int G, bool RIDE>
+static int launch_ml_depth(int depth, int grid, const int32_t* sig, const int32_t* nhm, const int32_t* gen,
+                           const int32_t* keys, const int32_t* key_idx, int nkeys,
+                           int32_t* status, int32_t* fout, int M, int* queue, const SubgroupRiders& riders,
+                           hipStream_t s) {
+  constexpr bool one = RIDE;
+  switch (depth) {
+    case 2:
+      if constexpr (ml_depth_ok<G>(2, one))
+        return launch_ml_kernel<G, RIDE, 2>(grid, sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue,
+                                            riders, s);
+      return -1;
+    case 1:
+      if constexpr (ml_depth_ok<G>(1, one))
+        return launch_ml_kernel<G, RIDE, 1>(grid, sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue,
+                                            riders, s);
+      return -1;
+    default:
+      return launch_ml_kernel<G, RIDE, 0>(grid, sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue,
+                                          riders, s);
+  }
+}
+// whether a launch runs the one-wave instantiation
+template <int G>
+static bool ml_one_wave(int* queue, int occ, int wg_per_cu, bool riders) {
+  if constexpr (rides<G>(0)) return riders || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W);
+  return false;
+}
+template <int G>
+int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth) {
+  int q = 0;
+  return ml_depth_for<G>(ml_one_wave<G>(queue ? &q : nullptr, queue ? grp_occ_ml<G>() : 0, wg_per_cu, riders), depth);
+}
+
 template <int G>
 int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
-                  const int32_t* key_idx, int nkeys, const int32_t* pk_items, int32_t* status, int32_t* fout, int M,
-                  int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
+                  const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout, int M,
+                  int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   int grid = 0;
@@ -860,16 +912,15 @@ int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, co
   // waves while another SIMD of the CU is empty; with the two-wave instantiation the
   // dispatcher doubles waves up, which measured 2.47 instead of 1.47 ms for the 12-lane
   // final exponentiation of 4096 items).
+  const bool one = ml_one_wave<G>(queue, occ, wg_per_cu, riders.per != 0);
+  const int d = ml_depth_for<G>(one, depth);
   if constexpr (rides<G>(0)) {
-    if (riders.per || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W)) {
-      hipLaunchKernelGGL((k_miller_grp<G, true>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen,
-                         keys, key_idx, nkeys, pk_items, status, fout, M, queue, riders);
-      return hipGetLastError() == hipSuccess ? 0 : -1;
-    }
+    if (one)
+      return launch_ml_depth<G, true>(d, grid, sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue,
+                                      riders, s);
   }
-  hipLaunchKernelGGL((k_miller_grp<G, false>), dim3(grid), dim3(Cfg<G>::THREADS), miller_lds_bytes<G>(), s, sig, nhm, gen, keys,
-                     key_idx, nkeys, pk_items, status, fout, M, queue, riders);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return launch_ml_depth<G, false>(d, grid, sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue,
+                                   riders, s);
 }
 template <int G>
 int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
@@ -898,8 +949,8 @@ int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_
 #define BLS_IS(g) || GRP_LANES == g
 #if GRP_KIND == 0 && (0 BLS_GRP_ML_LANES(BLS_IS))
 template int grp_launch_ml<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int,
-                                      const int32_t*, int32_t*, int32_t*, int, int*, int, const SubgroupRiders&,
-                                      hipStream_t);
+                                      int32_t*, int32_t*, int, int*, int, int, const SubgroupRiders&, hipStream_t);
+template int grp_ml_depth<GRP_LANES>(bool, int, bool, int);
 template int grp_occ_ml<GRP_LANES>();
 #endif
 #if GRP_KIND == 1 && (0 BLS_GRP_FE_LANES(BLS_IS))

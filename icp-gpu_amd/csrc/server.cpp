@@ -146,6 +146,7 @@ int blsgpu_server::launch(Slot& sl) {
   a.g_fe = c.g_fe;
   a.wpc_ml = c.wpc_ml;
   a.wpc_fe = c.wpc_fe;
+  a.depth_ml = c.depth_ml;
   a.queue = sl.queue.as<int>();
   a.comp = sl.comp.as<uint8_t>();
   a.nhm_ref = sl.nhm.as<uint64_t>();

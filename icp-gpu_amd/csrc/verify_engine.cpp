@@ -5,13 +5,12 @@
 
 namespace bls {
 
-static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, const int32_t* key_idx,
-                  const int32_t* pk_items, int32_t* status, hipStream_t s, const SubgroupRiders& riders = {}) {
+static int miller(const EngineArgs& a, const int32_t* gen, const int32_t* keys, const int32_t* key_idx, int nkeys,
+                  int32_t* status, hipStream_t s, const SubgroupRiders& riders = {}) {
   if (a.mode == 0)
-    return launch_miller(a.sig_pts, a.nhm_pts, gen, key_idx ? keys : pk_items, key_idx, a.nkeys, status, a.fbuf, a.M,
-                         a.block, s);
-  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, a.nkeys, pk_items, status, a.fbuf, a.M,
-                           a.queue && a.wpc_ml >= 0 ? a.queue : nullptr, a.wpc_ml, riders, s);
+    return launch_miller(a.sig_pts, a.nhm_pts, gen, keys, key_idx, nkeys, status, a.fbuf, a.M, a.block, s);
+  return launch_miller_grp(a.g_ml, a.sig_pts, a.nhm_pts, gen, keys, key_idx, nkeys, status, a.fbuf, a.M,
+                           a.queue && a.wpc_ml >= 0 ? a.queue : nullptr, a.wpc_ml, a.depth_ml, riders, s);
 }
 static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* sub_status, hipStream_t s,
                      const SubgroupRiders& riders = {}) {
@@ -28,7 +27,9 @@ int engine_legacy(const EngineArgs& a, hipStream_t s) {
   if (launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
   if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, s)) return -1;
   if (launch_lines_from_ref(a.pk_ref, a.pk_lines, a.M, s)) return -1;
-  if (miller(a, a.g2_lines, nullptr, nullptr, a.pk_lines, nullptr, s)) return -1;
+  // the M per-item tables are M keys, item i using key i
+  if (launch_iota(a.item_idx, a.M, s)) return -1;
+  if (miller(a, a.g2_lines, a.pk_lines, a.item_idx, a.M, nullptr, s)) return -1;
   return final_exp(a, nullptr, nullptr, s);
 }
 
@@ -56,7 +57,7 @@ static int launch_subgroup_fork(const EngineArgs& a, bool fork, hipStream_t s) {
 // (the final exponentiation then reads the subgroup status), else the final
 // exponentiation (its verdicts are corrected afterwards).
 static int engine_riders(const EngineArgs& a, const SubgroupRiders& rml, const SubgroupRiders& rfe, hipStream_t s) {
-  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s, rml)) return -1;
+  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, a.nkeys, a.status, s, rml)) return -1;
   if (mark(a, 3, s) || final_exp(a, a.status, rml.per ? a.sub_status : nullptr, s, rfe)) return -1;
   if (rfe.per && launch_apply_subgroup(a.results, a.gt, a.status, a.sub_status, a.M, s)) return -1;
   return mark(a, 4, s);
@@ -82,7 +83,7 @@ int engine_compressed(const EngineArgs& a, hipStream_t s) {
   const bool with_fe = late && fork && tiles_ml && tiles_fe && grp_fills_simds(a.g_ml, a.M) &&
                        !grp_fills_simds(a.g_fe, a.M);
   if (!with_fe && launch_subgroup_fork(a, fork, s)) return -1;
-  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s)) return -1;
+  if (mark(a, 2, s) || miller(a, a.gen_lines, a.keys, a.key_idx, a.nkeys, a.status, s)) return -1;
   if (with_fe && launch_subgroup_fork(a, fork, s)) return -1;
   if (fork && !late && hipStreamWaitEvent(s, a.join, 0) != hipSuccess) return -1;
   if (mark(a, 3, s) || final_exp(a, a.status, late ? nullptr : a.sub_status, s)) return -1;
@@ -100,11 +101,16 @@ int engine_prepare(const EngineArgs& a, hipStream_t s) {
 }
 int engine_miller(const EngineArgs& a, hipStream_t s) {
   if (a.queue && a.mode == 1 && launch_zero_words(a.queue, 1, s)) return -1;
-  return miller(a, a.gen_lines, a.keys, a.key_idx, nullptr, a.status, s);
+  return miller(a, a.gen_lines, a.keys, a.key_idx, a.nkeys, a.status, s);
 }
 int engine_final_exp(const EngineArgs& a, hipStream_t s) {
   if (a.queue && a.mode == 1 && launch_zero_words(a.queue + 1, 1, s)) return -1;
   return final_exp(a, a.status, a.sub_status, s);
+}
+
+int engine_miller_depth(const EngineArgs& a, bool riders) {
+  if (a.mode != 1) return 0;
+  return miller_grp_depth(a.g_ml, a.queue && a.wpc_ml >= 0, a.wpc_ml, riders, a.depth_ml);
 }
 
 int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s) {

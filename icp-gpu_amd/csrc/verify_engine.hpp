@@ -23,6 +23,7 @@ struct EngineArgs {
   int32_t* nhm_pts = nullptr;   // M x 28
   int32_t* g2_lines = nullptr;  // 5712 (legacy)
   int32_t* pk_lines = nullptr;  // legacy: the M items' internal line tables, 5712 words each
+  int32_t* item_idx = nullptr;  // legacy: M words, item i's table index (i)
   int32_t* status = nullptr;    // M (fused): decode status
   int32_t* sub_status = nullptr; // M (fused): subgroup status
   int32_t* fbuf = nullptr;      // M x 168 Miller values: word-major (SoA) in mode 0, item-major in mode 1
@@ -40,6 +41,8 @@ struct EngineArgs {
   int* queue = nullptr;
   int wpc_ml = 0;
   int wpc_fe = 0;
+  // key-line depth of the Miller loop (launch_miller_grp; -1 = the default)
+  int depth_ml = -1;
   // optional timing: events recorded before each phase and at the end (5 events)
   hipEvent_t* ev = nullptr;
   // second stream for the subgroup test, and the fork/join events (fused path)
@@ -59,5 +62,7 @@ int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s);
 int engine_prepare(const EngineArgs& a, hipStream_t s);
 int engine_miller(const EngineArgs& a, hipStream_t s);
 int engine_final_exp(const EngineArgs& a, hipStream_t s);
+// the key-line depth the Miller launch of a (with or without subgroup riders) runs
+int engine_miller_depth(const EngineArgs& a, bool riders);
 
 }  // namespace bls

@@ -179,6 +179,15 @@ int blsgpu_set_batched_engine(int g_miller, int g_final_exp, int wpc_miller, int
 int blsgpu_get_batched_engine(int M, int cfg[5]);
 /* out[0] graphs instantiated, out[1] graph launches */
 int blsgpu_batched_stats(int out[2]);
+/* Key-line depth of the lanes-per-item Miller loop (coefficient-tile depth; no
+ * counterpart in the reference): how many steps ahead each step's line of the item's key
+ * is read from memory into registers, 0 (read at the step that stores it to LDS), 1 or 2.
+ * The autotuner picks it for the Miller configuration it chooses.  set: force a depth on
+ * every form (-1 = back to the autotuned or default depth); a depth that the chosen
+ * kernel's register budget cannot hold runs as the largest one below it.  get: the depth
+ * a call of the plain form (form 1) or the batched form (form 0) of M items runs, or -1. */
+int blsgpu_set_line_depth(int depth);
+int blsgpu_get_line_depth(int form, int M);
 /* Autotuner on a device-resident job of M items, timed with HIP events (the median of three runs;
  * a candidate 25 % slower than the best so far on its first run is not run again).
  * It times every final-exponentiation candidate alone, then every Miller-loop candidate
@@ -190,6 +199,9 @@ int blsgpu_batched_stats(int out[2]);
  *   use the entry of the nearest class.
  * form 1 (plain): lanes-per-item candidates with one workgroup per tile; the fastest
  *   pair becomes the plain form's engine (as blsgpu_set_engine(1, ...)).
+ * Then, unless a depth is forced (blsgpu_set_line_depth), every key-line depth the chosen
+ * Miller kernel holds is timed the same way and the fastest is kept with the pair
+ * (blsgpu_get_line_depth reports it).
  * cfg_out (nullable) = {g_miller, g_final_exp, wpc_miller, wpc_final_exp}; ms_out
  * (nullable) = {best whole-sequence time, best final-exponentiation time} in ms.
  * Synchronous. */

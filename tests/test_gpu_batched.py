@@ -151,6 +151,37 @@ def test_autotuner_picks_a_valid_config_and_stays_exact(small):
     assert rc == 0, blsgpu.last_error()
     assert cfg[2] == -1 and cfg[3] == -1
     assert blsgpu.get_engine() == (1, int(cfg[0]), int(cfg[1]))
+    assert blsgpu.get_line_depth(1, 97) in (0, 1, 2)  # the tuned key-line depth
     r3, g3 = blsgpu.verify_compressed(small["sig"], small["nhm"], h[small["key_idx"]], want_gt=True)
     assert (r3 == small["want"]).all() and (g3 == small["wgt"]).all()
     blsgpu.set_engine(*saved)
+
+
+@pytest.mark.parametrize("depth", [0, 1, 2])
+def test_key_line_depths_match_oracle(large, depth):
+    """Every key-line depth (how many steps ahead the Miller loop reads its key lines into
+    registers) gives the oracle's verdicts and GT elements, in the plain form at 32, 12
+    (the one-wave kernel that carries the subgroup test) and 24 lanes and in the batched
+    form; the depth each launch runs is the requested one where its kernel holds it."""
+    print("case: key-line depth", depth, flush=True)
+    h = _register(large)
+    saved = blsgpu.get_engine()
+    n = 400
+    blsgpu.set_line_depth(depth)
+    try:
+        for eng, avail in (((1, 32, 12), 2), ((1, 12, 12), 2), ((1, 24, 12), 2), ((1, 16, 16), 0)):
+            blsgpu.set_engine(*eng)
+            assert blsgpu.get_line_depth(1, n) == min(depth, avail), eng
+            res, gt = blsgpu.verify_compressed(large["sig"][:n], large["nhm"][:n], h[large["key_idx"][:n]],
+                                               want_gt=True)
+            assert (res == large["want"][:n]).all(), eng
+            assert (gt == large["wgt"][:n]).all(), eng
+        blsgpu.set_batched_engine(32, 12, 0, 0, True)
+        assert blsgpu.get_line_depth(0, 1600) == depth
+        off = _split(1600, [700, 1, 899])
+        res, gt = _call_device(_device_run(large, h, off), off)
+        assert (res == large["want"]).all() and (gt == large["wgt"]).all()
+    finally:
+        blsgpu.set_line_depth(-1)
+        blsgpu.set_engine(*saved)
+        blsgpu.set_batched_engine(12, 16, 0, 0, True)
