@@ -22,9 +22,11 @@
  * against cuda/g2_coeffs.h:551-565 and cuda/test_bls_verify.cu:147; final
  * exponentiation against lib/test_cyc_sq.cu:141-153 plus bilinearity and
  * GT^r = 1; decompression by round trips, the Euler criterion and r*P = O by
- * brute force.  Parity unpinned: G1 from_compressed error classes beyond
- * accept/reject, identity handling inside multi_miller_loop, hash_to_curve,
- * and the reference's ChaCha20 key stream (the oracle uses its own seeded
+ * brute force; an identity signature in the fused path by e(O, G2) = 1 (its GT
+ * equals e(-H(m), pk) and the conjugate of e(sig, G2) for the valid signature).
+ * Parity unpinned: G1 from_compressed error classes beyond accept/reject,
+ * hash_to_curve against ic_bls12_381 itself (pinned by RFC 9380's vectors), and
+ * the reference's ChaCha20 key stream (the oracle uses its own seeded
  * generator, see orc_rand_*).
  */
 #ifndef BLS_ORACLE_H
