@@ -18,6 +18,7 @@
 
 #include "../../include/gpu_msm.h"
 #include "context.hpp"
+#include "host_copy.hpp"
 #include "kernels.hpp"
 #include "verify_engine.hpp"
 
@@ -142,6 +143,8 @@ int init_locked() {
   HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&g.ev_h2c, hipEventDisableTiming));
+  for (hipEvent_t& e : g.ev_ring) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (hipEvent_t& e : g.ev_part) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // the fixed workspace's device address on this device (g.device is current)
   void* sp = nullptr;
   HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -161,6 +164,31 @@ int init_locked() {
   return 0;
 }
 
+// Tables [first, first + n) of the caller's key-line array (2448 u64 each) to g.pk_ref on
+// the copy stream cs, through the pinned ring: host_copy fills a slot once the copy that
+// last read it has completed, then an asynchronous copy takes the slot to the device, so
+// that filling slot k overlaps the transfer of slot k - 1.
+static int stage_tables_locked(const uint64_t* pkc, int first, int n, hipStream_t cs) {
+  constexpr size_t tb = 2448 * 8;
+  constexpr int per = Ctx::kRingTables;
+  for (int c = first; c < first + n; c += per) {
+    const int k = first + n - c < per ? first + n - c : per;
+    const int slot = g.ring_next;
+    g.ring_next = (g.ring_next + 1) % Ctx::kRingSlots;
+    if (g.ring_used[slot]) HIPCHK(hipEventSynchronize(g.ev_ring[slot]));
+    uint8_t* h = g.ring.as<uint8_t>() + (size_t)slot * per * tb;
+    host_copy(h, pkc + (size_t)c * 2448, (size_t)k * tb);
+    HIPCHK(hipMemcpyAsync(g.pk_ref.as<uint8_t>() + (size_t)c * tb, h, (size_t)k * tb, hipMemcpyHostToDevice, cs));
+    HIPCHK(hipEventRecord(g.ev_ring[slot], cs));
+    g.ring_used[slot] = true;
+  }
+  return 0;
+}
+
+// Batches of at least this many items run the legacy sequence in two parts (the Miller
+// loop of the first half overlaps the copy of the second half's tables).
+constexpr int kLegacySplit = 1024;
+
 int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint64_t* g2c,
                          const uint64_t* pkc, int32_t* results, uint64_t* gt_out, int M) {
   CHK(init_locked());
@@ -178,10 +206,14 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   CHK(g.res.ensure(m * 4));
   CHK(g.fbuf.ensure(m * bls::MILLER_WORDS * 4));
   if (gt_out) CHK(g.gt.ensure(m * 72 * 8));
+  CHK(g.ring.ensure((size_t)Ctx::kRingSlots * Ctx::kRingTables * 2448 * 8));
   Staging st;
-  CHK(st.begin(2 * Staging::span(m * 96) + Staging::span(2448 * 8) + Staging::span(m * 2448 * 8) +
-               Staging::span(m * 4) + (gt_out ? Staging::span(m * 72 * 8) : 0)));
-  // with profiling on: events around the copies in, the kernels and the copies out
+  CHK(st.begin(2 * Staging::span(m * 96) + Staging::span(2448 * 8) + Staging::span(m * 4) +
+               (gt_out ? Staging::span(m * 72 * 8) : 0)));
+  // copies in and conversions on cs, pairing kernels and copies out on g.s
+  hipStream_t cs = g.s2;
+  // with profiling on: events at the first copy in, when the last key table has landed,
+  // after the kernels and after the copies out
   hipEvent_t ev[4] = {};
   if (g.profiling) {
     for (hipEvent_t& e : ev) {
@@ -192,13 +224,11 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
         HIPCHK(hipEventCreate(&e));
       }
     }
-    HIPCHK(hipEventRecord(ev[0], g.s));
+    HIPCHK(hipEventRecord(ev[0], cs));
   }
-  CHK(st.h2d(g.sig_ref.p, sigs, m * 96, g.s));
-  CHK(st.h2d(g.nhm_ref.p, nhms, m * 96, g.s));
-  CHK(st.h2d(g.g2_ref.p, g2c, 2448 * 8, g.s));
-  CHK(st.h2d(g.pk_ref.p, pkc, m * 2448 * 8, g.s));
-  if (ev[1]) HIPCHK(hipEventRecord(ev[1], g.s));
+  CHK(st.h2d(g.sig_ref.p, sigs, m * 96, cs));
+  CHK(st.h2d(g.nhm_ref.p, nhms, m * 96, cs));
+  CHK(st.h2d(g.g2_ref.p, g2c, 2448 * 8, cs));
   bls::EngineArgs a{};
   a.M = M;
   a.mode = g.mode;
@@ -217,11 +247,21 @@ int legacy_verify_locked(const uint64_t* sigs, const uint64_t* nhms, const uint6
   a.fbuf = g.fbuf.as<int32_t>();
   a.results = g.res.as<int32_t>();
   a.gt = gt_out ? g.gt.as<uint64_t>() : nullptr;
-  CHK(bls::engine_legacy(a, g.s));
+  CHK(bls::engine_legacy_setup(a, cs));
+  const int parts = g.mode == 1 && M >= kLegacySplit ? 2 : 1;
+  for (int p = 0, first = 0; p < parts; p++) {
+    const int n = p + 1 < parts ? M / parts : M - first;
+    CHK(stage_tables_locked(pkc, first, n, cs));
+    if (p + 1 == parts && ev[1]) HIPCHK(hipEventRecord(ev[1], cs));
+    CHK(bls::engine_legacy_part(a, first, n, cs, g.ev_part[p], g.s));
+    first += n;
+  }
+  CHK(bls::engine_legacy_finish(a, g.s));
   if (ev[2]) HIPCHK(hipEventRecord(ev[2], g.s));
   CHK(st.d2h(results, g.res.p, m * 4, g.s));
   if (gt_out) CHK(st.d2h(gt_out, g.gt.p, m * 72 * 8, g.s));
   if (ev[3]) HIPCHK(hipEventRecord(ev[3], g.s));
+  // every operation on cs precedes an event that g.s waited for
   HIPCHK(hipStreamSynchronize(g.s));
   st.finish();
   if (ev[0]) {

@@ -100,6 +100,14 @@ struct Ctx {
   DevBuf sig_ref, nhm_ref, g2_ref, pk_ref, comp, keyidx;
   DevBuf msg_bytes, msg_offs, dst_bytes;  // messages and DST of the hashing entry points
   PinnedBuf stage;                         // host staging of the blocking entry points
+  // the legacy entry point's key tables pass through a ring of pinned slots (kRingTables
+  // tables each): ev_ring[k] is recorded after the copy that reads slot k
+  static constexpr int kRingSlots = 3, kRingTables = 428;
+  PinnedBuf ring;
+  hipEvent_t ev_ring[kRingSlots] = {};
+  bool ring_used[kRingSlots] = {};
+  int ring_next = 0;
+  hipEvent_t ev_part[2] = {};  // a part of the legacy batch is in device memory
   DevBuf sig_pts, nhm_pts, g2_lines, pk_lines, status, sub_status, fbuf, res, gt;
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_h2c = nullptr;

@@ -32,6 +32,12 @@ struct Cfg {
 // one wavefront per SIMD, which the subgroup test's double-and-add uses.
 template <int G>
 constexpr bool rides(int which) { return which == 0 ? (G == 8 || G == 12) : (G == 9 || G == 12); }
+// Lane counts with an instantiation compiled for one wavefront per SIMD: those that carry
+// the subgroup test, and the 32-lane Miller loop, whose tile launches of up to 2048 items
+// (one wavefront per SIMD) would otherwise share SIMDs two waves to one while others
+// stay empty (the two-wave budget lets the dispatcher place them so).
+template <int G>
+constexpr bool one_wave_inst(int which) { return rides<G>(which) || (which == 0 && G == 32); }
 
 // Key-line depth of the Miller loop: how many steps ahead of its use each step's line of
 // the item's key is read from memory into registers (0: read at the step that stores it
@@ -57,9 +63,9 @@ template <int G>
 int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                   const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout, int M,
                   int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s);
-// the depth grp_launch_ml runs for these arguments
+// the depth grp_launch_ml runs for these arguments (M items)
 template <int G>
-int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth);
+int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth, int M);
 template <int G>
 int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
                   int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);

@@ -28,9 +28,9 @@ int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32
     default: return -1;
   }
 }
-int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth) {
+int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth, int M) {
   switch (G) {
-#define ML_CASE(g) case g: return grp_ml_depth<g>(queue, wg_per_cu, riders, depth);
+#define ML_CASE(g) case g: return grp_ml_depth<g>(queue, wg_per_cu, riders, depth, M);
     BLS_GRP_ML_LANES(ML_CASE)
 #undef ML_CASE
     default: return -1;

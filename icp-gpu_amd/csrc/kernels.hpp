@@ -55,7 +55,7 @@ struct SubgroupRiders {
 int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32_t* gen, const int32_t* keys,
                       const int32_t* key_idx, int nkeys, int32_t* status, int32_t* fout, int M,
                       int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s);
-int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth);
+int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth, int M);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
                          uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
 // Items per subgroup-test workgroup that a one-workgroup-per-tile launch of M items at G

@@ -696,6 +696,13 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
       const int e = w / NL, l = w % NL;
       g_lds[R0 + e * SLOT + l] = act ? fin[(size_t)it * (12 * NL) + w] : 0;
     }
+    // The shared inversion below reads the norm slot R3[2] of every item of the tile,
+    // and the ops store nothing for an inactive item: its slot would hold whatever an
+    // earlier workgroup or kernel left in this LDS (words outside the multiplier's input
+    // range corrupt the product, and so every item's inverse).  Zero counts as 1 there.
+    if (!act) {
+      for (int l = sub; l < NL; l += G) g_lds[R3 + 2 * SLOT + l] = 0;
+    }
   }
   __syncthreads();
 
@@ -885,16 +892,22 @@ static int launch_ml_depth(int depth, int grid, const int32_t* sig, const int32_
                                           riders, s);
   }
 }
-// whether a launch runs the one-wave instantiation
+// whether a launch of M items runs the one-wave instantiation: with riders, or when its
+// waves (one workgroup per tile, or the persistent grid) fit one on each SIMD
 template <int G>
-static bool ml_one_wave(int* queue, int occ, int wg_per_cu, bool riders) {
-  if constexpr (rides<G>(0)) return riders || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W);
+static bool ml_one_wave(int* queue, int occ, int wg_per_cu, bool riders, int M) {
+  if constexpr (one_wave_inst<G>(0)) {
+    if (riders) return true;
+    if (!queue) return (long)((M + Cfg<G>::IPW - 1) / Cfg<G>::IPW) * Cfg<G>::W <= 4L * grp_num_cus();
+    return at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W);
+  }
   return false;
 }
 template <int G>
-int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth) {
+int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth, int M) {
   int q = 0;
-  return ml_depth_for<G>(ml_one_wave<G>(queue ? &q : nullptr, queue ? grp_occ_ml<G>() : 0, wg_per_cu, riders), depth);
+  return ml_depth_for<G>(ml_one_wave<G>(queue ? &q : nullptr, queue ? grp_occ_ml<G>() : 0, wg_per_cu, riders, M),
+                         depth);
 }
 
 template <int G>
@@ -912,9 +925,9 @@ int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, co
   // waves while another SIMD of the CU is empty; with the two-wave instantiation the
   // dispatcher doubles waves up, which measured 2.47 instead of 1.47 ms for the 12-lane
   // final exponentiation of 4096 items).
-  const bool one = ml_one_wave<G>(queue, occ, wg_per_cu, riders.per != 0);
+  const bool one = ml_one_wave<G>(queue, occ, wg_per_cu, riders.per != 0, M);
   const int d = ml_depth_for<G>(one, depth);
-  if constexpr (rides<G>(0)) {
+  if constexpr (one_wave_inst<G>(0)) {
     if (one)
       return launch_ml_depth<G, true>(d, grid, sig, nhm, gen, keys, key_idx, nkeys, status, fout, M, queue,
                                       riders, s);
@@ -950,7 +963,7 @@ int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_
 #if GRP_KIND == 0 && (0 BLS_GRP_ML_LANES(BLS_IS))
 template int grp_launch_ml<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int,
                                       int32_t*, int32_t*, int, int*, int, int, const SubgroupRiders&, hipStream_t);
-template int grp_ml_depth<GRP_LANES>(bool, int, bool, int);
+template int grp_ml_depth<GRP_LANES>(bool, int, bool, int, int);
 template int grp_occ_ml<GRP_LANES>();
 #endif
 #if GRP_KIND == 1 && (0 BLS_GRP_FE_LANES(BLS_IS))

@@ -22,15 +22,34 @@ static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* 
                               a.queue && a.wpc_fe >= 0 ? a.queue + 1 : nullptr, a.wpc_fe, riders, s);
 }
 
-int engine_legacy(const EngineArgs& a, hipStream_t s) {
-  if (launch_points_from_ref(a.sig_ref, a.sig_pts, a.M, s)) return -1;
-  if (launch_points_from_ref(a.nhm_ref, a.nhm_pts, a.M, s)) return -1;
-  if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, s)) return -1;
-  if (launch_lines_from_ref(a.pk_ref, a.pk_lines, a.M, s)) return -1;
+int engine_legacy_setup(const EngineArgs& a, hipStream_t cs) {
+  if (launch_lines_from_ref(a.g2_ref, a.g2_lines, 1, cs)) return -1;
   // the M per-item tables are M keys, item i using key i
-  if (launch_iota(a.item_idx, a.M, s)) return -1;
-  if (miller(a, a.g2_lines, a.pk_lines, a.item_idx, a.M, nullptr, s)) return -1;
-  return final_exp(a, nullptr, nullptr, s);
+  return launch_iota(a.item_idx, a.M, cs);
+}
+int engine_legacy_part(const EngineArgs& a, int first, int n, hipStream_t cs, hipEvent_t ready, hipStream_t s) {
+  // thread-per-item Miller values are word-major with stride M: that mode runs one part
+  if (first < 0 || n <= 0 || first + n > a.M || (a.mode == 0 && n != a.M)) return -1;
+  // the part's points form SoA blocks of their own (stride n) at its items' offset
+  int32_t* sp = a.sig_pts + (size_t)first * POINT_WORDS;
+  int32_t* np = a.nhm_pts + (size_t)first * POINT_WORDS;
+  if (launch_points_from_ref(a.sig_ref + (size_t)first * 12, sp, n, cs)) return -1;
+  if (launch_points_from_ref(a.nhm_ref + (size_t)first * 12, np, n, cs)) return -1;
+  if (launch_lines_from_ref(a.pk_ref + (size_t)first * 2448, a.pk_lines + (size_t)first * KEY_TABLE_WORDS, n, cs))
+    return -1;
+  if (cs != s && (!ready || hipEventRecord(ready, cs) != hipSuccess || hipStreamWaitEvent(s, ready, 0) != hipSuccess))
+    return -1;
+  EngineArgs b = a;
+  b.M = n;
+  b.sig_pts = sp;
+  b.nhm_pts = np;
+  b.fbuf = a.fbuf + (size_t)first * MILLER_WORDS;
+  return miller(b, a.g2_lines, a.pk_lines, a.item_idx + first, a.M, nullptr, s);
+}
+int engine_legacy_finish(const EngineArgs& a, hipStream_t s) { return final_exp(a, nullptr, nullptr, s); }
+int engine_legacy(const EngineArgs& a, hipStream_t s) {
+  if (engine_legacy_setup(a, s) || engine_legacy_part(a, 0, a.M, s, nullptr, s)) return -1;
+  return engine_legacy_finish(a, s);
 }
 
 static inline int mark(const EngineArgs& a, int k, hipStream_t s) {
@@ -110,7 +129,7 @@ int engine_final_exp(const EngineArgs& a, hipStream_t s) {
 
 int engine_miller_depth(const EngineArgs& a, bool riders) {
   if (a.mode != 1) return 0;
-  return miller_grp_depth(a.g_ml, a.queue && a.wpc_ml >= 0, a.wpc_ml, riders, a.depth_ml);
+  return miller_grp_depth(a.g_ml, a.queue && a.wpc_ml >= 0, a.wpc_ml, riders, a.depth_ml, a.M);
 }
 
 int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s) {

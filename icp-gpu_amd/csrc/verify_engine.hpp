@@ -54,6 +54,16 @@ struct EngineArgs {
 };
 
 int engine_legacy(const EngineArgs& a, hipStream_t s);
+// The legacy sequence in parts, so that the Miller loop of one part runs while the next
+// part's tables are still being copied in: engine_legacy_setup (on the copy stream cs:
+// the generator's table and the item-index array), then per part of items
+// [first, first + n) engine_legacy_part (its points and tables converted on cs, `ready`
+// recorded there, then its Miller loop on s after that event), then
+// engine_legacy_finish (the final exponentiation of all M on s).  Mode 0 runs one part.
+// engine_legacy is the three on one stream.
+int engine_legacy_setup(const EngineArgs& a, hipStream_t cs);
+int engine_legacy_part(const EngineArgs& a, int first, int n, hipStream_t cs, hipEvent_t ready, hipStream_t s);
+int engine_legacy_finish(const EngineArgs& a, hipStream_t s);
 int engine_compressed(const EngineArgs& a, hipStream_t s);
 // decode (and optionally subgroup-check) a.comp into a.sig_pts / a.status
 int engine_decompress(const EngineArgs& a, int check_subgroup, hipStream_t s);

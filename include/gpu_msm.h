@@ -34,7 +34,9 @@ int gpu_crypto_init(void);
  * Verifies e(sigs[i], G2) * e(neg_hms[i], pk_i) == 1 for i < M, with pk_i given by its
  * 68 precomputed lines pk_coeffs + i*2448 and the generator's lines in g2_coeffs.
  * results[i] = 1 (valid) or 0 (invalid).  Returns 0, or -1 on any device error or
- * M <= 0 (the reference's M = 0 launch also fails). */
+ * M <= 0 (the reference's M = 0 launch also fails).  The key tables (19,584 B per item)
+ * are copied through pinned staging by the calling thread and three library threads;
+ * the arrays are only read, and only during the call. */
 int gpu_batch_bls_verify(const uint64_t* sigs, const uint64_t* neg_hms, const uint64_t* g2_coeffs,
                          const uint64_t* pk_coeffs, int32_t* results, int M);
 
@@ -222,8 +224,11 @@ int blsgpu_get_engine(int* mode, int* g_miller, int* g_final_exp);
 int blsgpu_set_profiling(int on);
 int blsgpu_phase_times(double ms[4], int* calls);
 /* The same for gpu_batch_bls_verify / blsgpu_batch_verify_gt calls made while profiling
- * is on: ms[0] host-to-device copies of the caller's arrays, ms[1] the kernels, ms[2]
- * the device-to-host copies of the results, summed over *calls calls. */
+ * is on: ms[0] from the first host-to-device copy of the caller's arrays until the last
+ * key table has landed, ms[1] from then until the kernels end (batches of 1024 items or
+ * more run the Miller loop of their first half during the copies of the second half,
+ * so part of the kernel time falls in ms[0]), ms[2] the device-to-host copies of the
+ * results, summed over *calls calls. */
 int blsgpu_legacy_phase_times(double ms[3], int* calls);
 
 /* Last error message of the calling thread ("" if none). */

@@ -26,9 +26,9 @@ ORACLE_THREADS = 16
 
 
 def _oracle(w):
-    coeffs = np.array([O.g2_prepare(p) for p in w["pk"]])
-    g2c = O.g2_prepare(O.g2_gen())
-    return O.verify_compressed_batch(w["sig"], w["nhm"], g2c, coeffs, w["key_idx"], threads=ORACLE_THREADS)
+    w["coeffs"] = np.array([O.g2_prepare(p) for p in w["pk"]])
+    w["g2c"] = O.g2_prepare(O.g2_gen())
+    return O.verify_compressed_batch(w["sig"], w["nhm"], w["g2c"], w["coeffs"], w["key_idx"], threads=ORACLE_THREADS)
 
 
 @pytest.fixture(scope="module")
@@ -79,6 +79,39 @@ def test_uniform_4096_plain_form(uniform, engine):
         torch.cuda.synchronize()
         assert (d_res.cpu().numpy() == w["want"]).all()
         assert (d_gt.cpu().numpy().view(np.uint64) == w["wgt"]).all()
+    finally:
+        blsgpu.set_engine(*saved)
+
+
+@pytest.mark.parametrize("engine", [(1, 32, 12), (1, 12, 12), (0, 16, 16)],
+                         ids=["ml32_fe12_two_parts", "ml12_fe12_two_parts", "thread_per_item_one_part"])
+def test_uniform_4096_legacy_abi(uniform, engine):
+    """gpu_batch_bls_verify as a replica calls it: host arrays of uncompressed signatures
+    (zeros where the encoding does not decode: the flagless identity), -H(m), the
+    generator's lines and 2448 u64 of lines per item.  In the lanes-per-item mode the
+    4096 items run as two parts of 2048, the Miller loop of the first during the copies
+    of the second, whose key tables cross the three-slot pinned ring in five chunks each
+    (slots reused); the thread-per-item mode runs one part.  Compared with the oracle's
+    verification of the same arrays, every verdict and GT element."""
+    print("case: uniform 4096, legacy ABI, engine", engine, flush=True)
+    w = uniform
+    if "legacy" not in w:
+        unc, _ = blsgpu.decompress_status(w["sig"], check_subgroup=False)
+        pkc = np.ascontiguousarray(w["coeffs"][w["key_idx"]])
+        ores, ogt = O.verify_batch(unc, w["nhm"], w["g2c"], pkc, threads=ORACLE_THREADS)
+        w["legacy"] = (unc, pkc, ores, ogt)
+    unc, pkc, ores, ogt = w["legacy"]
+    # every valid item verifies; the others (wrong message, zeros, off-subgroup points) fail
+    assert (ores == (w["cat"] == 0)).all()
+    saved = blsgpu.get_engine()
+    try:
+        blsgpu.set_engine(*engine)
+        res, gt = blsgpu.batch_verify(unc, w["nhm"], w["g2c"], pkc, want_gt=True)
+        assert (res == ores).all(), np.nonzero(res != ores)
+        assert (gt == ogt).all(), np.nonzero((gt != ogt).any(axis=1))
+        # the same arrays again (the ring's slots now hold the previous call's tables)
+        res2 = blsgpu.batch_verify(unc, w["nhm"], w["g2c"], pkc)
+        assert (res2 == ores).all()
     finally:
         blsgpu.set_engine(*saved)
 

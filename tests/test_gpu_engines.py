@@ -62,6 +62,27 @@ def test_legacy_path(cfg, data):
     assert (gt == ogt).all()
 
 
+@pytest.mark.parametrize("cfg", [(1, 32, 12), (1, 16, 16)], ids=lambda c: "mode%d_ml%d_fe%d" % c)
+def test_legacy_two_parts_odd_size(cfg):
+    """1031 items (over the 1024 at which the lanes-per-item legacy sequence splits the
+    batch): parts of 515 and 516 items, each one full and one partial chunk of the pinned
+    ring; every verdict and GT element against the oracle."""
+    print("case: legacy two parts, 1031 items, engine", cfg, flush=True)
+    n = 1031
+    sks, pks, coeffs = O.make_keys(47, 7)
+    key_idx = np.array([(i * 3) % 7 for i in range(n)], np.int32)
+    inv = {0: "wrong_msg", 514: "wrong_msg", 515: "non_subgroup", 700: "no_sqrt", 1030: "wrong_msg"}
+    comp, unc, nhm, _ = O.make_batch(47, n, sks, key_idx, inv)
+    g2c = O.g2_prepare(O.g2_gen())
+    pkc = np.ascontiguousarray(coeffs[key_idx])
+    ores, ogt = O.verify_batch(unc, nhm, g2c, pkc, threads=16)
+    assert ores.sum() == n - len(inv)
+    blsgpu.set_engine(*cfg)
+    res, gt = blsgpu.batch_verify(unc, nhm, g2c, pkc, want_gt=True)
+    assert (res == ores).all(), np.nonzero(res != ores)
+    assert (gt == ogt).all(), np.nonzero((gt != ogt).any(axis=1))
+
+
 def test_bad_key_index_is_reported():
     print("case: bad key index", flush=True)
     blsgpu.set_engine(1, 16, 16)
