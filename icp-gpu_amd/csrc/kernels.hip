@@ -63,53 +63,6 @@ struct LinesAoS {  // one key table (same for every lane or chosen per lane)
 };
 // ---- kernels ----
 
-// legacy ABI points: M x 12 u64 (x | y, reference Montgomery form) -> internal SoA
-__global__ void k_points_from_ref(const u64* __restrict__ in, i32* __restrict__ out, int M) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  st_fp_soa(out, 0, M, i, ld_ref(in + (size_t)i * 12));
-  st_fp_soa(out, NL, M, i, ld_ref(in + (size_t)i * 12 + 6));
-}
-
-// legacy ABI line tables: M x 2448 u64 (68 x 6 Fp, reference form) -> internal tables,
-// one after the other (the key-major layout of the resident keys).  Both sides are flat
-// arrays of field elements (element g at in + 6 g and out + 14 g), so a workgroup takes
-// LINES_PER_WG consecutive elements: their input words are read into LDS with one
-// coalesced load per lane and word, each thread converts one element, and the output
-// words leave LDS as contiguous 16-byte stores (a thread's own 14 words are 56 bytes
-// apart from its neighbours', which wrote partial cache lines from every store).
-constexpr int LINES_PER_WG = 256;
-__global__ __launch_bounds__(LINES_PER_WG) void k_lines_from_ref(const u64* __restrict__ in, i32* __restrict__ out,
-                                                                 int M) {
-  __shared__ u64 win[LINES_PER_WG * 6];
-  __shared__ __align__(16) i32 wout[LINES_PER_WG * NL];
-  const size_t n = (size_t)M * (NUM_COEFFS * 6);
-  const size_t g0 = (size_t)blockIdx.x * LINES_PER_WG;
-  const int cnt = n - g0 < (size_t)LINES_PER_WG ? (int)(n - g0) : LINES_PER_WG;
-  const u64* src = in + g0 * 6;
-  for (int k = threadIdx.x; k < cnt * 6; k += LINES_PER_WG) win[k] = src[k];
-  __syncthreads();
-  if ((int)threadIdx.x < cnt) {
-    u32 w[12];
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-      const u64 v = win[threadIdx.x * 6 + j];
-      w[2 * j] = (u32)v;
-      w[2 * j + 1] = (u32)(v >> 32);
-    }
-    const Fp v = from_ref(w);
-#pragma unroll
-    for (int l = 0; l < NL; l++) wout[threadIdx.x * NL + l] = v.l[l];
-  }
-  __syncthreads();
-  // out + 14 g0 is 16-byte aligned (g0 is a multiple of 256; the launcher checks out)
-  const int words = cnt * NL;
-  int4* d4 = reinterpret_cast<int4*>(out + g0 * NL);
-  const int4* s4 = reinterpret_cast<const int4*>(wout);
-  for (int k = threadIdx.x; k < words / 4; k += LINES_PER_WG) d4[k] = s4[k];
-  for (int k = (words & ~3) + threadIdx.x; k < words; k += LINES_PER_WG) out[g0 * NL + k] = wout[k];
-}
-
 // G2Prepared of K affine points given in reference form (24 u64: x.c0 x.c1 y.c0 y.c1)
 // -> key-major internal tables (K x 5712 words)
 __global__ __launch_bounds__(64, 1) void k_g2_prepare(const u64* __restrict__ q, i32* __restrict__ out, int K) {
@@ -253,16 +206,6 @@ __global__ void k_table_to_ref(const i32* __restrict__ t, u64* __restrict__ out)
 // ---- launch wrappers (host) ----
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-int launch_points_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s) {
-  hipLaunchKernelGGL(k_points_from_ref, dim3(nblk(M, 64)), dim3(64), 0, s, in, out, M);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-int launch_lines_from_ref(const uint64_t* in, int32_t* out, int M, hipStream_t s) {
-  if (M <= 0 || (reinterpret_cast<uintptr_t>(out) & 15)) return -1;
-  long long n = (long long)M * NUM_COEFFS * 6;
-  hipLaunchKernelGGL(k_lines_from_ref, dim3(nblk(n, LINES_PER_WG)), dim3(LINES_PER_WG), 0, s, in, out, M);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
 int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s) {
   hipLaunchKernelGGL(k_table_to_ref, dim3(nblk(NUM_COEFFS * 6, 64)), dim3(64), 0, s, t, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
