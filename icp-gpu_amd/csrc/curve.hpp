@@ -288,9 +288,16 @@ __device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
 // X == beta x Z^2 and Y == -y Z^3 (the test of g1_in_subgroup).
 // The field products go through FpCalls: the test is latency work on few wavefronts,
 // and one copy of the multiplier keeps its loop small.
+// An item that did not decode tests the generator instead, its result ignored: every
+// lane of the wavefront runs the whole test (a wavefront with 8 or fewer active lanes
+// issues about three times slower per instruction on this GPU).
 __device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32* status, int M, int i) {
-  if (status[i] != DEC_OK) return DEC_OK;
-  const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
+  const bool decoded = status[i] == DEC_OK;
+  Fp x = fp_const(G1_GEN_X_M), y = fp_const(G1_GEN_Y_M);
+  if (decoded) {
+    x = ld_soa(pts, 0, M, i);
+    y = ld_soa(pts, NL, M, i);
+  }
   Jac<Fp> p1 = {x, y, fp_one()};
   Jac<Fp> a = mul_absx_jac<FpCalls>(p1);
   Jac<Fp> b = mul_absx_jac<FpCalls>(a);
@@ -300,7 +307,7 @@ __device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32*
     in = eq(b.x, FpCalls::mul(FpCalls::mul(x, fp_const(BETA_M)), z2)) &&
          eq(b.y, neg(FpCalls::mul(y, FpCalls::mul(z2, b.z))));
   }
-  return in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
+  return in || !decoded ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }
 // Decode 48 big-endian bytes given as 12 big-endian-packed 32-bit words (w[0] holds bytes
 // 0..3).  Semantics of ic_bls12_381 G1Affine::from_compressed: the reference GPU kernel

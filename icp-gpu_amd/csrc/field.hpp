@@ -336,21 +336,25 @@ __device__ __forceinline__ Fp inv(const Fp& a) { return pow_fixed(a, INV_EXP); }
 // multiplications instead of 384 and 229): odd powers a, a^3, .., a^15 first, then one
 // multiplication per window.  The window walk is the same on every lane (the exponent is
 // a constant), so the digit selection is a scalar branch around register copies and
-// the loop holds a single multiplier body.
+// the loop holds a single multiplier body.  The odd powers are eight named values, not
+// an array: a table indexed by the (uniform, but not constant) digit went to private
+// memory, one scratch load per limb at every window.
 template <int LEN>
 __device__ __forceinline__ Fp pow_win(const Fp& a, const uint8_t (&win)[LEN][2]) {
-  Fp pw[8];
-  pw[0] = a;
   const Fp a2 = sqr(a);
-#pragma unroll
-  for (int k = 1; k < 8; k++) pw[k] = mul(pw[k - 1], a2);
-  // every index below is a compile-time constant, so pw[] stays in registers
+  const Fp p1 = a, p3 = mul(p1, a2), p5 = mul(p3, a2), p7 = mul(p5, a2);
+  const Fp p9 = mul(p7, a2), p11 = mul(p9, a2), p13 = mul(p11, a2), p15 = mul(p13, a2);
   auto pick = [&](int d) -> Fp {
-    Fp t = pw[0];
-#pragma unroll
-    for (int k = 1; k < 8; k++)
-      if ((d >> 1) == k) t = pw[k];
-    return t;
+    switch (__builtin_amdgcn_readfirstlane(d) >> 1) {
+      case 0: return p1;
+      case 1: return p3;
+      case 2: return p5;
+      case 3: return p7;
+      case 4: return p9;
+      case 5: return p11;
+      case 6: return p13;
+      default: return p15;
+    }
   };
   Fp r = pick(win[0][1]);
 #pragma unroll 1

@@ -65,16 +65,22 @@ struct LinesAoS {  // one key table (same for every lane or chosen per lane)
 
 // G2Prepared of K affine points given in reference form (24 u64: x.c0 x.c1 y.c0 y.c1)
 // -> key-major internal tables (K x 5712 words)
+// Lanes past the last key (here and in k_g2_decompress, k_g1_decompress) repeat it and
+// store nothing: a wavefront with 8 or fewer active lanes issues about three times
+// slower per instruction on this GPU, which would triple a single key's registration.
 __global__ __launch_bounds__(64, 1) void k_g2_prepare(const u64* __restrict__ q, i32* __restrict__ out, int K) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= K) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < K;
+  const int i = live ? t : K - 1;
   const u64* s = q + (size_t)i * 24;
   Aff<Fp2> Q = {{ld_ref(s), ld_ref(s + 6)}, {ld_ref(s + 12), ld_ref(s + 18)}};
   i32* dst = out + (size_t)i * KEY_WORDS;
   g2_prepare(Q, [&](int k, const Line& L) {
     const Fp* f = &L.c0.c0;
+    if (live) {
 #pragma unroll
-    for (int e = 0; e < 6; e++) st_fp_aos(dst + k * LINE_WORDS + e * NL, f[e]);
+      for (int e = 0; e < 6; e++) st_fp_aos(dst + k * LINE_WORDS + e * NL, f[e]);
+    }
   });
 }
 
@@ -83,8 +89,9 @@ __global__ __launch_bounds__(64, 1) void k_g2_prepare(const u64* __restrict__ q,
 // with the subgroup test (check_subgroup != 0).  One thread per key.
 __global__ __launch_bounds__(64, 1) void k_g2_decompress(const uint8_t* __restrict__ in, u64* __restrict__ q_ref, i32* __restrict__ status,
                                 int K, int check_subgroup) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= K) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < K;
+  const int i = live ? t : K - 1;
   u32 be[24];
   const uint8_t* b = in + (size_t)i * 96;
 #pragma unroll
@@ -92,6 +99,7 @@ __global__ __launch_bounds__(64, 1) void k_g2_decompress(const uint8_t* __restri
     be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
   Aff<Fp2> Q;
   const int32_t st = g2_decompress(be, Q, check_subgroup != 0);
+  if (!live) return;
   u64* o = q_ref + (size_t)i * 24;
   if (st == DEC_OK) {
     st_ref(o, Q.x.c0);
@@ -107,8 +115,9 @@ __global__ __launch_bounds__(64, 1) void k_g2_decompress(const uint8_t* __restri
 // compressed G1 (M x 48 bytes) -> internal SoA points + status per item
 __global__ __launch_bounds__(64, 1) void k_g1_decompress(const uint8_t* __restrict__ in, i32* __restrict__ pts, i32* __restrict__ status,
                                 int M, int check_subgroup) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < M;
+  const int i = live ? t : M - 1;
   u32 be[12];
   const uint8_t* b = in + (size_t)i * 48;
 #pragma unroll
@@ -116,6 +125,7 @@ __global__ __launch_bounds__(64, 1) void k_g1_decompress(const uint8_t* __restri
     be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
   Aff<Fp> P;
   int32_t st = g1_decompress(be, P, check_subgroup != 0);
+  if (!live) return;
   st_fp_soa(pts, 0, M, i, P.x);
   st_fp_soa(pts, NL, M, i, P.y);
   status[i] = st;

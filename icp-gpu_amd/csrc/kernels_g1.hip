@@ -18,10 +18,16 @@ __device__ __forceinline__ void st_soa(i32* base, int w0, int M, int i, const Fp
   for (int j = 0; j < NL; j++) base[(size_t)(w0 + j) * M + i] = v.l[j];
 }
 
+// Lanes past the last item decode it again and store nothing (here and in the other
+// thread-per-item kernels): a wavefront with 8 or fewer active lanes issues about three
+// times slower per instruction on this GPU (a dependent chain of Fp products measured
+// 6,020 cycles per product against 2,018 with 16 lanes or more), so a small batch's one
+// partial wavefront would take three times as long.
 __global__ __launch_bounds__(64, 1) void k_g1_decode(const uint8_t* __restrict__ in, i32* __restrict__ pts,
                                                   i32* __restrict__ status, int M) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < M;
+  const int i = live ? t : M - 1;
   u32 be[12];
   const uint8_t* b = in + (size_t)i * 48;
 #pragma unroll
@@ -29,6 +35,7 @@ __global__ __launch_bounds__(64, 1) void k_g1_decode(const uint8_t* __restrict__
     be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
   Aff<Fp> P;
   const int32_t st = g1_decompress(be, P, false);
+  if (!live) return;
   st_soa(pts, 0, M, i, P.x);
   st_soa(pts, NL, M, i, P.y);
   status[i] = st;
@@ -39,17 +46,21 @@ __global__ __launch_bounds__(64, 1) void k_g1_decode(const uint8_t* __restrict__
 // doubling instead of seven); the additions and the comparison run on both lanes, and
 // the even lane writes the status.  Both lanes of a pair take every branch together
 // (they hold the same values), so the partner exchange always finds its partner active.
+// Pairs past the last item repeat it, and items that did not decode test the generator
+// instead, with no result taken from either: every lane runs the whole test (see
+// k_g1_decode).
 __global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
                                                     i32* __restrict__ sub_status, int M) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t >> 1;
+  const bool live = (t >> 1) < M;
+  const int i = live ? t >> 1 : M - 1;
   const bool odd = t & 1;
-  if (i >= M) return;
-  if (status[i] != DEC_OK) {
-    if (!odd) sub_status[i] = DEC_OK;
-    return;
+  const bool decoded = status[i] == DEC_OK;
+  Fp x = fp_const(G1_GEN_X_M), y = fp_const(G1_GEN_Y_M);
+  if (decoded) {
+    x = ld_soa(pts, 0, M, i);
+    y = ld_soa(pts, NL, M, i);
   }
-  const Fp x = ld_soa(pts, 0, M, i), y = ld_soa(pts, NL, M, i);
   const Jac<Fp> p1 = {x, y, fp_one()};
   const Jac<Fp> b = mul_absx_pair(mul_absx_pair(p1, odd), odd);
   bool in = false;
@@ -57,7 +68,7 @@ __global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ p
     const Fp z2 = sqr(b.z);
     in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
   }
-  if (!odd) sub_status[i] = in ? DEC_OK : DEC_NOT_IN_SUBGROUP;
+  if (live && !odd) sub_status[i] = in || !decoded ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }
 
 // status := sub_status where status is OK (for entry points that report decoding alone)

@@ -616,10 +616,15 @@ __device__ __forceinline__ int next_tile(int* queue, int round) {
 // membership of G1 (the subgroup test needs only the decoded signature, and no pairing
 // value needs it).  These workgroups run at the default wave priority on the SIMDs the
 // tiles leave free.
+// Lanes with no item of their own repeat the workgroup's last one and store nothing, so
+// that every lane runs the test (a wavefront with few active lanes issues slower).
 __device__ __forceinline__ void subgroup_rider(const i32* pts, const i32* status, i32* out, int M, int per, int k) {
   const int t = (int)threadIdx.x;
   const int i = k * per + t;
-  if (t < per && i < M) out[i] = g1_subgroup_status(pts, status, M, i);
+  const bool live = t < per && i < M;
+  const int last = (k * per + per < M ? k * per + per : M) - 1;
+  const int32_t s = g1_subgroup_status(pts, status, M, live ? i : last);
+  if (live) out[i] = s;
 }
 
 template <int G, bool RIDE, int D>
@@ -711,29 +716,36 @@ __device__ __forceinline__ void final_exp_tile(int tile, const i32* __restrict__
   run_op<G, WPE>(sched::OP_N6A, sub, act, R1, 0, R2, T, K);            // R2[0..5] = (t0, t1, t2)
   run_op<G, WPE>(sched::OP_N6B, sub, act, R1, R2, R3, T, K);           // R3[0..1] = sc
   run_op<G, WPE>(sched::OP_N2, sub, act, R3, 0, R3 + 2 * SLOT, T, K);  // R3[2] = n
-  // one inversion per workgroup (Montgomery's trick over its IPW items): thread 0 forms
-  // the prefix products of the n's (inactive or zero entries count as 1), inverts the
-  // total by batched divsteps (inv_fp_uniform) and peels the individual inverses off
-  // into R3[3] of each item
-  if (threadIdx.x == 0) {
-    Fp pre[IPW];
-    Fp accp = fp_one();
-    for (int q = 0; q < IPW; q++) {
-      const int bq = q * FE_SLOTS * SLOT + 36 * SLOT;
-      Fp nq = lds_ld(bq + 2 * SLOT);
-      if (is_zero(nq)) nq = fp_one();
-      pre[q] = accp;
-      accp = mul(accp, nq);
+  // One inversion per workgroup (Montgomery's trick over its IPW items), on every lane:
+  // code that a handful of lanes of a wavefront run alone issues about three times slower
+  // per instruction on this GPU (a dependent chain of products measured 6,020 cycles per
+  // product with 8 active lanes or fewer, 2,018 with 16 or more), so no step below runs
+  // on fewer than half a wavefront.
+  //  (a) the lanes of each item test its n for zero (inactive items; a zero counts as 1
+  //      in the products and gets a zero inverse);
+  //  (b) lanes 0..31 of each wave form the prefix products P_q = n_0 .. n_(q-1) into
+  //      R3[6] of item q, lanes 32..63 the suffix products S_q = n_(q+1) .. n_(IPW-1)
+  //      into R3[7], one product per step on both halves;
+  //  (c) every lane inverts the product of all n by batched divsteps (inv_fp_uniform:
+  //      the values are the same on every lane, both halves' products canonicalised);
+  //  (d) item q's inverse inv * P_q * S_q goes to R3[3].
+  {
+    const Fp n_own = lds_ld(R3 + 2 * SLOT);
+    const bool z = is_zero(n_own);
+    __syncthreads();
+    if (in_wg && sub == 0 && z) lds_st(R3 + 2 * SLOT, fp_one());
+    __syncthreads();
+    const bool pre = (threadIdx.x & 63) < 32;
+    Fp acc = fp_one();
+    for (int k = 0; k < IPW; k++) {
+      const int bq = (pre ? k : IPW - 1 - k) * FE_SLOTS * SLOT + 36 * SLOT;
+      if ((threadIdx.x & 31) == 0 && threadIdx.x < 64) lds_st(bq + (pre ? 6 : 7) * SLOT, acc);
+      acc = mul(acc, lds_ld(bq + 2 * SLOT));
     }
-    Fp inv_all = inv_fp_uniform(accp);  // thread 0 alone
-    for (int q = IPW - 1; q >= 0; q--) {
-      const int bq = q * FE_SLOTS * SLOT + 36 * SLOT;
-      Fp nq = lds_ld(bq + 2 * SLOT);
-      const bool z = is_zero(nq);
-      if (z) nq = fp_one();
-      lds_st(bq + 3 * SLOT, z ? fp_zero() : mul(inv_all, pre[q]));
-      inv_all = mul(inv_all, nq);
-    }
+    const Fp inv_all = inv_fp_uniform(acc);
+    __syncthreads();
+    const Fp inv_own = mul(mul(inv_all, lds_ld(R3 + 6 * SLOT)), lds_ld(R3 + 7 * SLOT));
+    if (in_wg && sub == 0) lds_st(R3 + 3 * SLOT, z ? fp_zero() : inv_own);
   }
   __syncthreads();
   run_op<G, WPE>(sched::OP_INV2, sub, act, R3, R3 + 3 * SLOT, R3 + 4 * SLOT, T, K);  // R3[4..5] = sc^-1
