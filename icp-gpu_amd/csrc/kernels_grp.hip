@@ -353,21 +353,27 @@ __device__ __forceinline__ void static_round(GRec* __restrict__ desc, int sub, b
   const u32 dst = rec[0] & 0xffffu;
   // a wavefront none of whose lanes has a product or output in this round skips it
   const bool work = __ballot(dst != sched::IDLE) != 0;
+  // In the Miller loop a lane with no product or output in the round (or of an inactive
+  // item) sits it out: measured 1.6 % faster (1.276 -> 1.250 ms per 4,096 items at 32
+  // lanes).  In the final exponentiation it runs the round on its padding record, which
+  // measured 4 % faster than masking it.
+  constexpr bool MASK = GRP_KIND == 0;
+  const bool busy = work && (!MASK || (active && dst != sched::IDLE));
   if constexpr (RI < o.np) {
-    if (work) {
+    if (busy) {
       Fp a, b;
       if constexpr (rd.flags & 1) a = load_term<1, C::sr.ra>(rec, R);
       else a = sum_terms<1, rd.na, C::sr.ra, C::RW, (C::sr.wa > 2)>(rec, R);
       if constexpr (rd.flags & 2) b = load_term<sched::B_AT, C::sr.rb>(rec, R);
       else b = sum_terms<sched::B_AT, rd.nb, C::sr.rb, C::RW, (C::sr.wb > 2)>(rec, R);
       const Fp pr = mul(a, b);
-      if (active && dst != sched::IDLE) lds_st(R.t + (int)dst * SLOT, pr);
+      if (MASK || (active && dst != sched::IDLE)) lds_st(R.t + (int)dst * SLOT, pr);
     }
     if constexpr (RI + 1 == o.np) __syncthreads();
   } else {
-    if (work) {
+    if (busy) {
       const Fp v = fold_top(sum_terms<1, rd.na, C::sr.ra>(rec, R));
-      if (active && dst != sched::IDLE) lds_st(D + (int)dst * SLOT, v);
+      if (MASK || (active && dst != sched::IDLE)) lds_st(D + (int)dst * SLOT, v);
     }
   }
   if constexpr (RI + 1 < C::NR) static_round<G, OP, RI + 1>(desc, sub, active, R, D, nxt);
@@ -670,6 +676,8 @@ __device__ __noinline__ void cyc_exp_grp(int sub, bool act, int src, int dst, in
   run_op<G, WPE>(sched::OP_COPY, sub, act, src, 0, dst, T, K);
   FirstRec<G, sched::OP_CYC_SQR> sq0;  // the same every iteration
   sq0.load(sub);
+  // (the product's first record is read at each product: held across the loop it made
+  // the 12-lane kernel 7 % slower, 1.48 -> 1.58 ms per 4,096 items)
   for (int i = 62; i >= 0; i--) {
     run_op_c<G, sched::OP_CYC_SQR>(sq0, sub, act, dst, 0, dst, T, K);
     if ((BLS_X >> i) & 1) run_op_c<G, sched::OP_MUL>(sub, act, dst, src, dst, T, K);
