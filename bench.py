@@ -379,11 +379,23 @@ def measure_decompress(lib, blsgpu, points48, steps):
             if lib.gpu_g1_decompress_batch(_p(src), _p(dst), n):
                 raise RuntimeError(blsgpu.last_error())
         ms = (time.perf_counter() - t0) * 1e3 / steps
-        out.append({"points": n, "ms_per_call": ms, "points_per_s": n / (ms * 1e-3), "steps": steps})
+        # the same points without the subgroup test (the reference's decoder checks
+        # nothing): blsgpu_g1_decompress_status with check_subgroup = 0
+        st = np.zeros(n, np.int32)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if lib.blsgpu_g1_decompress_status(_p(src), _p(dst), _p(st), n, 0):
+                raise RuntimeError(blsgpu.last_error())
+        ms_nosub = (time.perf_counter() - t0) * 1e3 / steps
+        if not (st == 0).all():
+            raise RuntimeError("a valid point was rejected")
+        out.append({"points": n, "ms_per_call": ms, "points_per_s": n / (ms * 1e-3), "steps": steps,
+                    "ms_per_call_without_subgroup_test": ms_nosub})
     return {"runs": out,
-            "workload": "valid compressed signatures of the uniform batch; blocking host entry point, copies "
-                        "included; the reference publishes 448 points in 0.98 ms on an RTX PRO 6000 "
-                        "(README.md:243), without the validity checks"}
+            "workload": "valid compressed signatures of the uniform batch; blocking host entry points, copies "
+                        "included: gpu_g1_decompress_batch (every check, the subgroup test included), and "
+                        "blsgpu_g1_decompress_status without the subgroup test; the reference publishes "
+                        "448 points in 0.98 ms on an RTX PRO 6000 (README.md:243), with none of the checks"}
 
 
 def measure_keys(lib, blsgpu, pk, steps):
