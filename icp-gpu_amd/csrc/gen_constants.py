@@ -267,6 +267,7 @@ def main(out_path=None):
     arr("FROB12_0", to_int(g12[0]), "(1+u)^((p-1)/6) = (this, .)")
     arr("FROB12_1", to_int(g12[1]), "(1+u)^((p-1)/6) = (., this)")
     arr("BETA_M", to_int(beta), "cube root of unity: (beta x, y) = [-x^2](x, y) on G1")
+    arr("HALF_M", to_int((P + 1) // 2), "1/2 (the square root in Fp2 by the norm)")
     arr("G1_GEN_X_M", to_int(gx), "G1 generator x (the stand-in point of lanes with no item to test)")
     arr("G1_GEN_Y_M", to_int(gy), "G1 generator y")
     arr("B_G2_M", to_int(4), "G2 curve constant b' = 4(1+u) = (this, this)")

@@ -71,6 +71,18 @@ def _special_encodings():
         nb = bytearray(xb)
         nb[0] ^= 0x20
         add("off subgroup %d, other root" % i, nb)
+    # x = x0 + 2u with Im(x^3) = x1 (3 x0^2 - x1^2) = -4, so that y^2 = x^3 + 4(1 + u) lies
+    # in Fp: the square root's zero-imaginary-part case.  The two roots x0 give y^2 a
+    # square in Fp (y real) and a non-square (y purely imaginary); both sort flags.
+    r = (4 - 4 * pow(2, -1, O.P)) * pow(3, -1, O.P) % O.P
+    assert pow(r, (O.P - 1) // 2, O.P) == 1
+    x0 = pow(r, (O.P + 1) // 4, O.P)
+    for x0s in (x0, O.P - x0):
+        real = pow((x0s ** 3 - 12 * x0s + 4) % O.P, (O.P - 1) // 2, O.P) == 1
+        for sflag in (0, 0x20):
+            xb = bytearray((2).to_bytes(48, "big") + x0s.to_bytes(48, "big"))
+            xb[0] |= 0x80 | sflag
+            add("y^2 in Fp, y %s, sort flag %d" % ("real" if real else "imaginary", sflag >> 5), xb)
     return names, np.stack(enc)
 
 

@@ -76,6 +76,35 @@ def test_solvability_and_off_subgroup_points():
     assert seen[0] > 4 and seen[-3] > 4
 
 
+def test_y_squared_in_fp():
+    """x = x0 + 2u with Im(x^3) = -4, so y^2 = x^3 + 4(1 + u) has no u part: the decoded
+    y is real when y^2 is a square in Fp and purely imaginary otherwise; it squares back
+    to y^2 and follows the sort flag (the case the device's square root by the norm
+    handles apart)"""
+    r = (4 - 4 * pow(2, -1, P)) * pow(3, -1, P) % P
+    x0 = pow(r, (P + 1) // 4, P)
+    assert x0 * x0 % P == r
+    kinds = set()
+    for x0s in (x0, P - x0):
+        y2 = (x0s ** 3 - 12 * x0s + 4) % P  # Re(x^3) + 4 with x1 = 2
+        assert (3 * x0s * x0s * 2 - 8 + 4) % P == 0  # Im(x^3) + 4 = 0
+        for sflag in (0, 0x20):
+            enc = bytearray((2).to_bytes(48, "big") + x0s.to_bytes(48, "big"))
+            enc[0] |= 0x80 | sflag
+            rc, q = O.g2_decompress(np.frombuffer(bytes(enc), np.uint8), check=False)
+            assert rc == 0 and O.g2_is_on_curve(q)
+            y0, y1 = O.unmont(q[12:18]), O.unmont(q[18:24])
+            if pow(y2, (P - 1) // 2, P) == 1:
+                assert y1 == 0 and y0 * y0 % P == y2
+                kinds.add("real")
+            else:
+                assert y0 == 0 and (P - y1 * y1) % P == y2
+                kinds.add("imaginary")
+            largest = y1 > (P - 1) // 2 or (y1 == 0 and y0 > (P - 1) // 2)
+            assert largest == bool(sflag)
+    assert kinds == {"real", "imaginary"}
+
+
 def test_rejected_encodings():
     g = O.g2_gen()
     enc = bytearray(O.g2_compress(g))
