@@ -23,7 +23,15 @@ __device__ __forceinline__ void st_soa(i32* base, int w0, int M, int i, const Fp
 // times slower per instruction on this GPU (a dependent chain of Fp products measured
 // 6,020 cycles per product against 2,018 with 16 lanes or more), so a small batch's one
 // partial wavefront would take three times as long.
-__global__ __launch_bounds__(64, 1) void k_g1_decode(const uint8_t* __restrict__ in, i32* __restrict__ pts,
+// Above 256 wavefronts, workgroups of four wavefronts: the four go to the four SIMDs of
+// one CU, so a launch of up to 1,024 wavefronts keeps one per SIMD.  With one-wavefront
+// workgroups (212 registers, so two fit a SIMD) the dispatcher doubled them up on some
+// SIMDs while others stayed empty: 34,368 items decoded in 0.58-0.70 ms instead of 0.38
+// (batched form 21.27 -> 21.11 ms, 65,536 items 39.48 -> 38.91 ms).  Up to 256
+// wavefronts, one-wavefront workgroups spread over that many CUs: four to a CU made the
+// 4096-item decode 1 % slower.
+constexpr int DECODE_BLOCK = 256;
+__global__ __launch_bounds__(DECODE_BLOCK, 1) void k_g1_decode(const uint8_t* __restrict__ in, i32* __restrict__ pts,
                                                   i32* __restrict__ status, int M) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = t < M;
@@ -114,7 +122,8 @@ int launch_iota(int32_t* p, int n, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hipStream_t s) {
-  hipLaunchKernelGGL(k_g1_decode, dim3((M + 63) / 64), dim3(64), 0, s, in, pts, status, M);
+  const int block = M <= 256 * 64 ? 64 : DECODE_BLOCK;
+  hipLaunchKernelGGL(k_g1_decode, dim3((M + block - 1) / block), dim3(block), 0, s, in, pts, status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s) {
