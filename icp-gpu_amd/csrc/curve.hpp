@@ -222,15 +222,31 @@ __device__ __forceinline__ Jac<Fp> mul_absx_jac(const Jac<Fp>& q) {
   }
   return acc;
 }
-// ---- [|x|] Q on a lane pair ----
-// (the cofactor chain of the hash to G1 and the subgroup test of decoded signatures)
-// Both lanes of a pair (lanes 2k, 2k + 1) hold the same point.  A doubling's seven field
-// products run as three steps of one product per lane plus one the pair computes twice,
-// each lane reading its partner's result by DPP (quad_perm 1,0,3,2: lane i reads i ^ 1).
-__device__ __forceinline__ Fp pair_swap(const Fp& a) {
+// ---- [|x|] P on a lane quad: complete projective formulas ----
+// (the cofactor chain of the hash to G1 and the standalone subgroup test)
+// Homogeneous coordinates (X : Y : Z) with x = X / Z, y = Y / Z; the identity is
+// (0 : 1 : 0).  The complete formulas of Renes, Costello and Batina (eprint 2015/1060,
+// algorithm 9 for a doubling and algorithm 7 for an addition, a = 0, b3 = 3b = 12) hold
+// for every pair of points (P + P, P + (-P), the identity), and their products come in
+// two steps of independent products.  The four lanes of a quad (lanes 4k .. 4k + 3, all
+// holding the same points) take one product each per step and read the others' by DPP:
+// a doubling is two product latencies (Y^2 | YZ | Z^2 | XY, then four products of
+// those) where dbl-2009-l on one lane is seven, an addition four (six products per
+// step, the last two of each step computed twice).  All four lanes of a quad take every
+// branch together (their values are the same), so a DPP read always finds its source
+// lane active.
+__device__ __forceinline__ Fp pair_swap(const Fp& a) {  // lane i reads lane i ^ 1
   Fp r;
 #pragma unroll
   for (int j = 0; j < NL; j++) r.l[j] = __builtin_amdgcn_update_dpp(a.l[j], a.l[j], 0xB1, 0xF, 0xF, false);
+  return r;
+}
+template <int K>
+__device__ __forceinline__ Fp quad_get(const Fp& a) {  // every lane reads lane K of its quad
+  constexpr int ctrl = K | (K << 2) | (K << 4) | (K << 6);
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < NL; j++) r.l[j] = __builtin_amdgcn_update_dpp(a.l[j], a.l[j], ctrl, 0xF, 0xF, false);
   return r;
 }
 __device__ __forceinline__ Fp pick(bool c, const Fp& a, const Fp& b) {
@@ -239,33 +255,88 @@ __device__ __forceinline__ Fp pick(bool c, const Fp& a, const Fp& b) {
   for (int j = 0; j < NL; j++) r.l[j] = c ? a.l[j] : b.l[j];
   return r;
 }
-// dbl-2009-l (the formulas of jdbl) with the products split over the pair; odd = the
-// lane's parity.  Step 1: A = X^2 | B = Y^2; step 2: C = B^2 | T = (X + B)^2;
-// step 3: Y Z | F = E^2; then E (D - X3) on both lanes.
-__device__ __forceinline__ Jac<Fp> jdbl_pair(const Jac<Fp>& p, bool odd) {
-  const Fp s1 = sqr(pick(odd, p.x, p.y)), s1o = pair_swap(s1);
-  const Fp A = pick(odd, s1, s1o), B = pick(odd, s1o, s1);
-  const Fp s2 = sqr(pick(odd, B, carry(add(p.x, B)))), s2o = pair_swap(s2);
-  const Fp C = pick(odd, s2, s2o), T = pick(odd, s2o, s2);
-  const Fp D = norm(dbl(sub(sub(T, A), C)));
-  const Fp E = norm(add(dbl(A), A));
-  const Fp s3 = mul(pick(odd, p.y, E), pick(odd, p.z, E)), s3o = pair_swap(s3);
-  const Fp YZ = pick(odd, s3, s3o), F = pick(odd, s3o, s3);
-  Jac<Fp> r;
-  r.x = norm(sub(F, dbl(D)));
-  const Fp C8 = norm(dbl(dbl(dbl(C))));
-  r.y = norm(sub(mul(E, sub(D, r.x)), C8));
-  r.z = norm(dbl(YZ));
-  return r;
+__device__ __forceinline__ Fp pick4(int q, const Fp& a0, const Fp& a1, const Fp& a2, const Fp& a3) {
+  return pick(q & 2, pick(q & 1, a3, a2), pick(q & 1, a1, a0));
 }
-// [|x|] q on a lane pair (as mul_absx_jac; the additions run on both lanes)
-__device__ __forceinline__ Jac<Fp> mul_absx_pair(const Jac<Fp>& q, bool odd) {
-  Jac<Fp> acc = q;
+// c0 a0 + c1 a1 (+ c2 a2) for N1 values and small integer coefficients (|c| <= 128,
+// uniform or per lane): one carry pass in 64 bits, then the fold of the top limb.  Limbs
+// 0..12 sum to below 3 * 2^35; the top limb to below 3 * 2^25.4 < 2^31: N1 out.
+__device__ __forceinline__ Fp lin3(const Fp& a0, int c0, const Fp& a1, int c1, const Fp& a2, int c2) {
+  Fp r;
+  i64 acc = 0;
+#pragma unroll
+  for (int i = 0; i < NL - 1; i++) {
+    acc += (i64)c0 * a0.l[i] + (i64)c1 * a1.l[i] + (i64)c2 * a2.l[i];
+    r.l[i] = (i32)acc & MASK28;
+    acc >>= 28;
+  }
+  r.l[NL - 1] = (i32)(acc + (i64)c0 * a0.l[NL - 1] + (i64)c1 * a1.l[NL - 1] + (i64)c2 * a2.l[NL - 1]);
+  return fold_top(r);
+}
+__device__ __forceinline__ Fp lin2(const Fp& a0, int c0, const Fp& a1, int c1) {
+  Fp r;
+  i64 acc = 0;
+#pragma unroll
+  for (int i = 0; i < NL - 1; i++) {
+    acc += (i64)c0 * a0.l[i] + (i64)c1 * a1.l[i];
+    r.l[i] = (i32)acc & MASK28;
+    acc >>= 28;
+  }
+  r.l[NL - 1] = (i32)(acc + (i64)c0 * a0.l[NL - 1] + (i64)c1 * a1.l[NL - 1]);
+  return fold_top(r);
+}
+
+struct Prj {
+  Fp x, y, z;
+};
+// 2P on the quad (q = the lane's index in it).  X, Z in N1 or N2, Y in N2; out: X, Z
+// N1, Y N2 (a sum of two products).
+//   step 1: Y^2 | YZ | Z^2 | XY
+//   step 2: 8 b3 Y^2 Z^2 | 8 Y^3 Z | (Y^2 - 3 b3 Z^2)(Y^2 + b3 Z^2) | 2 XY (Y^2 - 3 b3 Z^2)
+//   X3 = the fourth, Y3 = the first + the third, Z3 = the second
+__device__ __forceinline__ Prj quad_dbl(const Prj& p, int q) {
+  const Fp s1 = mul(pick(q == 3, p.x, pick(q == 2, p.z, p.y)), pick(q == 1 || q == 2, p.z, p.y));
+  const Fp y2 = quad_get<0>(s1), yz = quad_get<1>(s1), z2 = quad_get<2>(s1), xy = quad_get<3>(s1);
+  // the lane's operands: c0 Y^2 + c1 YZ + c2 Z^2 and d0 Y^2 + d1 Z^2 + d2 XY
+  const int c0 = q >= 2 ? 1 : 0, c1 = q == 1 ? 8 : 0, c2 = q == 0 ? 96 : (q >= 2 ? -36 : 0);
+  const int d0 = q == 3 ? 0 : 1, d1 = q == 2 ? 12 : 0, d2 = q == 3 ? 2 : 0;
+  const Fp s2 = mul(lin3(y2, c0, yz, c1, z2, c2), lin3(y2, d0, z2, d1, xy, d2));
+  return {quad_get<3>(s2), add(quad_get<0>(s2), quad_get<2>(s2)), quad_get<1>(s2)};
+}
+// P + R on the quad, N1 coordinates in and out.
+//   step 1: t0 = X1 X2 | t1 = Y1 Y2 | t2 = Z1 Z2 | t3 = (X1 + Y1)(X2 + Y2), then
+//           t4 = (Y1 + Z1)(Y2 + Z2) | t5 = (X1 + Z1)(X2 + Z2) on lanes 0 | 1 (2 | 3 repeat)
+//   with xy = t3 - t0 - t1, yz = t4 - t1 - t2, xz = t5 - t0 - t2:
+//   step 2: A = yz b3 xz | B = xy (t1 - b3 t2) | C = b3 xz 3 t0 | D = (t1 - b3 t2)(t1 + b3 t2),
+//           then E = 3 t0 xy | F = (t1 + b3 t2) yz
+//   X3 = B - A, Y3 = D + C, Z3 = F + E
+__device__ __forceinline__ Prj quad_add(const Prj& p, const Prj& r, int q) {
+  const bool h = q & 1;
+  const Fp s1 = mul(pick4(q, p.x, p.y, p.z, add(p.x, p.y)), pick4(q, r.x, r.y, r.z, add(r.x, r.y)));
+  const Fp s1b = mul(pick(h, add(p.x, p.z), add(p.y, p.z)), pick(h, add(r.x, r.z), add(r.y, r.z)));
+  const Fp t0 = quad_get<0>(s1), t1 = quad_get<1>(s1), t2 = quad_get<2>(s1), t3 = quad_get<3>(s1);
+  const Fp t4 = quad_get<0>(s1b), t5 = quad_get<1>(s1b);
+  const Fp xy = lin3(t3, 1, t0, -1, t1, -1), yz = lin3(t4, 1, t1, -1, t2, -1);
+  const Fp xz3 = lin3(t5, 12, t0, -12, t2, -12);  // b3 xz
+  const Fp ym = lin2(t1, 1, t2, -12), yp = lin2(t1, 1, t2, 12), x3 = lin2(t0, 3, t0, 0);
+  const Fp s2 = mul(pick4(q, yz, xy, xz3, ym), pick4(q, xz3, ym, x3, yp));
+  const Fp s2b = mul(pick(h, yp, x3), pick(h, yz, xy));
+  const Fp A = quad_get<0>(s2), B = quad_get<1>(s2), C = quad_get<2>(s2), D = quad_get<3>(s2);
+  const Fp E = quad_get<0>(s2b), F = quad_get<1>(s2b);
+  return {norm(sub(B, A)), norm(add(D, C)), norm(add(F, E))};
+}
+// [|x|] P on the quad (P in N1), N1 out
+__device__ __forceinline__ Prj quad_mul_absx(const Prj& p, int q) {
+  Prj acc = p;
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
-    acc = jdbl_pair(acc, odd);
-    if ((BLS_X >> i) & 1) acc = jadd_jac<FieldInline>(acc, q);
+    acc = quad_dbl(acc, q);
+    if ((BLS_X >> i) & 1) {
+      acc.y = norm(acc.y);
+      acc = quad_add(acc, p, q);
+    }
   }
+  acc.y = norm(acc.y);
   return acc;
 }
 

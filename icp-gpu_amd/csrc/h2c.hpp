@@ -1,12 +1,12 @@
 // Hash to G1 on the device: RFC 9380, suite BLS12381G1_XMD:SHA-256_SSWU_RO_ (what
 // ic_bls12_381's HashToCurve<ExpandMsgXmd<Sha256>> computes; the reference hashes on
 // the CPU before its GPU step, ic-integration/patches/ic-gpu-consensus.patch:454-457,
-// with the DST of ic-bench/src/main.rs:68).  One thread per message:
+// with the DST of ic-bench/src/main.rs:68).  Four lanes per message (kernels_h2c.hip):
 //   expand_message_xmd (SHA-256) -> two field elements (64 bytes each, mod p)
 //   -> simplified SWU onto E': y^2 = x^3 + A'x + B' (x kept as a fraction, y affine
 //      from one exponentiation) -> the 11-isogeny to E (rational maps evaluated
-//      homogeneously, result in Jacobian coordinates)
-//   -> Q0 + Q1 -> [1 - x](Q0 + Q1) (cofactor clearing) -> affine.
+//      homogeneously, result in projective coordinates)
+//   -> Q0 + Q1 -> [1 - x](Q0 + Q1) (cofactor clearing, complete formulas) -> affine.
 #pragma once
 #include "curve.hpp"
 
@@ -145,13 +145,24 @@ __device__ __forceinline__ u32 sgn0(const Fp& a) { return (u32)to_plain(a).l[0] 
 // xd = Z A' when t = 0); g(x1) = U / V with U = xn^3 + A' xn xd^2 + B' xd^3, V = xd^3.
 // y1 = (U V^3)^((p-3)/4) U V squares to U / V exactly when U / V is a square, and to
 // -U / V otherwise; then x2 = Z u^2 x1 and y2 = y1 Z u^3 sqrt(-Z) (g(x2) = Z^3 u^6 g(x1)).
-// The isogeny's maps, evaluated at x = xn / xd as homogeneous polynomials
-// XN, XD, YN, YD of degrees 11, 10, 15, 15, give x = XN / (xd XD) and y = y YN / YD;
-// the Jacobian result takes Z = T YD with T = xd XD: X = XN T YD^2, Y = y YN T^3 YD^2.
-// (out of line, by value: one copy of the exponentiation serves both field elements;
-// field products inline, measured 12 % faster than through FpCalls here)
+// The isogeny's maps, evaluated at x = xn / xd as forms of degree 15 (the x map's
+// numerator and denominator, of degrees 11 and 10, padded with leading zero
+// coefficients: XN xd^4 and XD xd^5), give x = XN xd^4 / (XD xd^5) = XN / (xd XD) and
+// y = y YN / YD, so the projective result is X = XN' YD, Y = y YN XD', Z = XD' YD (XN',
+// XD' the padded forms); Z = 0 (a point of the isogeny's kernel) gives the identity
+// (0 : 1 : 0).  On a lane quad holding one message: lanes 0 | 1 map u0 | u1 and evaluate
+// the x map's two forms, lanes 2 | 3 the same field elements and the y map's, and the
+// pairs (0, 2), (1, 3) exchange their forms and split the final products.
+// (out of line, by value; field products inline, measured 12 % faster than through
+// FpCalls here)
+__device__ __forceinline__ Fp quad_swap2(const Fp& a) {  // lane i reads lane i ^ 2
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < NL; j++) r.l[j] = __builtin_amdgcn_update_dpp(a.l[j], a.l[j], 0x4E, 0xF, 0xF, false);
+  return r;
+}
 template <class M>
-__device__ __noinline__ Jac<Fp> map_to_g1_curve(Fp u) {
+__device__ __noinline__ Prj map_to_g1_curve(Fp u, int q) {
   const Fp A = fp_const(SSWU_A_M), B = fp_const(SSWU_B_M), Zc = fp_const(SSWU_Z_M);
   const Fp zu2 = M::mul(Zc, M::sqr(u));
   const Fp t = norm(add(M::sqr(zu2), zu2));
@@ -174,22 +185,30 @@ __device__ __noinline__ Jac<Fp> map_to_g1_curve(Fp u) {
     y = M::mul(M::mul(y1, M::mul(zu2, u)), fp_const(SSWU_SQRT_MZ_M));
   }
   if (sgn0(y) != sgn0(u)) y = neg(y);
-  // the isogeny: one pass over the powers xd^j, j = 1..15, shared by the four maps
+  // the isogeny: numerator N and denominator D of the lane's map, one pass over the
+  // powers xd^j, j = 1..15 (coefficient 15 - j at step j)
+  const bool ymap = q >= 2;
   Fp pw = fp_one();
-  Fp XN = fp_const(ISO_XNUM[11]), XD = fp_one(), YN = fp_const(ISO_YNUM[15]), YD = fp_one();
+  Fp N = pick(ymap, fp_const(ISO_YNUM[15]), fp_zero()), D = pick(ymap, fp_one(), fp_zero());
 #pragma unroll 1
   for (int j = 1; j <= 15; j++) {
+    const int k = 15 - j;
+    Fp cxn = fp_zero(), cxd = fp_zero();
+    if (k <= 11) cxn = fp_const(ISO_XNUM[k]);
+    if (k <= 9) cxd = fp_const(ISO_XDEN[k]);
+    if (k == 10) cxd = fp_one();
     pw = M::mul(pw, xd);
-    YN = norm(add(M::mul(YN, x), M::mul(fp_const(ISO_YNUM[15 - j]), pw)));
-    YD = norm(add(M::mul(YD, x), M::mul(fp_const(ISO_YDEN[15 - j]), pw)));
-    if (j <= 11) XN = norm(add(M::mul(XN, x), M::mul(fp_const(ISO_XNUM[11 - j]), pw)));
-    if (j <= 10) XD = norm(add(M::mul(XD, x), M::mul(fp_const(ISO_XDEN[10 - j]), pw)));
+    N = norm(add(M::mul(N, x), M::mul(pick(ymap, fp_const(ISO_YNUM[k]), cxn), pw)));
+    D = norm(add(M::mul(D, x), M::mul(pick(ymap, fp_const(ISO_YDEN[k]), cxd), pw)));
   }
-  const Fp T = M::mul(xd, XD), YD2 = M::sqr(YD);
-  Jac<Fp> r;
-  r.z = M::mul(T, YD);
-  r.x = M::mul(M::mul(XN, T), YD2);
-  r.y = M::mul(M::mul(M::mul(y, YN), M::mul(M::sqr(T), T)), YD2);
+  const Fp No = quad_swap2(N), Do = quad_swap2(D);
+  const Fp XN = pick(ymap, No, N), XD = pick(ymap, Do, D), YN = pick(ymap, N, No), YD = pick(ymap, D, Do);
+  // lanes 0, 1: XN' YD, then XD' YD; lanes 2, 3: y YN, then y YN XD'
+  const Fp s1 = M::mul(pick(ymap, y, XN), pick(ymap, YN, YD));
+  const Fp s2 = M::mul(pick(ymap, s1, XD), pick(ymap, XD, YD));
+  const Fp s1o = quad_swap2(s1), s2o = quad_swap2(s2);
+  Prj r = {pick(ymap, s1o, s1), pick(ymap, s2, s2o), pick(ymap, s2o, s2)};
+  if (is_zero(r.z)) r = {fp_zero(), fp_one(), fp_zero()};
   return r;
 }
 
@@ -202,15 +221,14 @@ __device__ __forceinline__ void hash_to_field2(const uint8_t* msg, int len, cons
   u1 = field_from_512(ub[2], ub[3]);
 }
 // [1 - x](Q0 + Q1) (cofactor clearing, sec. 7 with h_eff of sec. 8.8.1) as affine
-// (x, y) in internal form, on both lanes of a pair; false for the identity
-__device__ __forceinline__ bool clear_cofactor_affine(const Jac<Fp>& q0, const Jac<Fp>& q1, bool odd, Fp& x,
-                                                      Fp& y) {
-  const Jac<Fp> r = jadd_jac<FieldInline>(q0, q1);
-  const Jac<Fp> h = jadd_jac<FieldInline>(mul_absx_pair(r, odd), r);  // [|x| + 1] R = [1 - x] R
+// (x, y) in internal form, on every lane of the quad; false for the identity
+__device__ __forceinline__ bool clear_cofactor_affine(const Prj& q0, const Prj& q1, int q, Fp& x, Fp& y) {
+  const Prj r = quad_add(q0, q1, q);
+  const Prj h = quad_add(quad_mul_absx(r, q), r, q);  // [|x| + 1] R = [1 - x] R
   if (is_zero(h.z)) return false;
-  const Fp zi = inv_fp(h.z), zi2 = FpCalls::sqr(zi);
-  x = FpCalls::mul(h.x, zi2);
-  y = FpCalls::mul(h.y, FpCalls::mul(zi2, zi));
+  const Fp zi = inv_fp(h.z);
+  x = mul(h.x, zi);
+  y = mul(h.y, zi);
   return true;
 }
 

@@ -1,7 +1,7 @@
 // G1 signature decoding, split in two kernels so that the subgroup test can run
 // concurrently with the Miller loop (which needs only the decoded coordinates):
 //   k_g1_decode   flags, x < p, y = sqrt(x^3 + 4) with the sign choice -> point + status
-//   k_g1_subgroup [x^2] P == (beta x, -y) on decoded points -> subgroup status (two
+//   k_g1_subgroup [x^2] P == (beta x, -y) on decoded points -> subgroup status (four
 //                 lanes per item)
 // Semantics as ic_bls12_381 G1Affine::from_compressed; the reference GPU decoder
 // (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the decoding steps
@@ -49,34 +49,32 @@ __global__ __launch_bounds__(DECODE_BLOCK, 1) void k_g1_decode(const uint8_t* __
   status[i] = st;
 }
 
-// The test of g1_subgroup_status on a lane pair: lanes 2k and 2k + 1 of the grid hold
-// item k and split the doublings of [|x|]([|x|]P) (jdbl_pair: four product latencies per
-// doubling instead of seven); the additions and the comparison run on both lanes, and
-// the even lane writes the status.  Both lanes of a pair take every branch together
-// (they hold the same values), so the partner exchange always finds its partner active.
-// Pairs past the last item repeat it, and items that did not decode test the generator
+// The test of g1_subgroup_status on a lane quad: lanes 4k .. 4k + 3 of the grid hold
+// item k and split the products of [|x|]([|x|]P) (quad_mul_absx: complete projective
+// formulas, two product latencies per doubling where dbl-2009-l on one lane takes
+// seven); [x^2]P == (beta x, -y) is then tested projectively, X == beta x Z and
+// Y == -y Z with Z != 0, on every lane, and lane 0 of the quad writes the status.
+// Quads past the last item repeat it, and items that did not decode test the generator
 // instead, with no result taken from either: every lane runs the whole test (see
-// k_g1_decode).
+// k_g1_decode), and the four lanes of a quad always run together.
 __global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ pts, const i32* __restrict__ status,
                                                     i32* __restrict__ sub_status, int M) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = (t >> 1) < M;
-  const int i = live ? t >> 1 : M - 1;
-  const bool odd = t & 1;
+  const bool live = (t >> 2) < M;
+  const int i = live ? t >> 2 : M - 1;
+  const int q = t & 3;
   const bool decoded = status[i] == DEC_OK;
   Fp x = fp_const(G1_GEN_X_M), y = fp_const(G1_GEN_Y_M);
   if (decoded) {
     x = ld_soa(pts, 0, M, i);
     y = ld_soa(pts, NL, M, i);
   }
-  const Jac<Fp> p1 = {x, y, fp_one()};
-  const Jac<Fp> b = mul_absx_pair(mul_absx_pair(p1, odd), odd);
+  Prj b = {x, y, fp_one()};
+#pragma unroll 1
+  for (int k = 0; k < 2; k++) b = quad_mul_absx(b, q);
   bool in = false;
-  if (!is_zero(b.z)) {
-    const Fp z2 = sqr(b.z);
-    in = eq(b.x, mul(mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(mul(y, mul(z2, b.z))));
-  }
-  if (live && !odd) sub_status[i] = in || !decoded ? DEC_OK : DEC_NOT_IN_SUBGROUP;
+  if (!is_zero(b.z)) in = eq(b.x, mul(mul(x, fp_const(BETA_M)), b.z)) && eq(b.y, neg(mul(y, b.z)));
+  if (live && q == 0) sub_status[i] = in || !decoded ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }
 
 // status := sub_status where status is OK (for entry points that report decoding alone)
@@ -127,7 +125,7 @@ int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hi
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s) {
-  hipLaunchKernelGGL(k_g1_subgroup, dim3((2 * (size_t)M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
+  hipLaunchKernelGGL(k_g1_subgroup, dim3((4 * (size_t)M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_merge_status(int32_t* status, const int32_t* sub_status, int M, hipStream_t s) {

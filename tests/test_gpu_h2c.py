@@ -102,8 +102,9 @@ def test_rejects_bad_message_arguments():
 
 
 def test_large_batch_on_curve_and_sampled_against_oracle():
-    # 3,000 messages (94 workgroups of 32 lane pairs): every output on E and in the
-    # r-torsion ([r]P checked on a sample), a sample against the oracle
+    # 3,000 messages (188 workgroups of 16 lane quads, the last one half full): every
+    # output on E and in the r-torsion ([r]P checked on a sample), a sample against the
+    # oracle
     print("case: hash_to_g1 n=3000", flush=True)
     n = 3000
     msgs = [bytes(O.rand_bytes(97, 8_000_000 + i, 32)) for i in range(n)]
