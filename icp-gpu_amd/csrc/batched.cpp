@@ -386,12 +386,40 @@ int blsgpu_autotune_device(const uint8_t* d_sigs48, const uint64_t* d_neg_hms, c
   // and the best of a few runs can be a run where the concurrent kernel happened to
   // interfere less than it does in steady state).  A candidate whose first timed run is
   // 25 % slower than `bound` (the best so far) is not run again.
+  // For the batched form with graphs on, the whole sequence runs from a graph of it, as
+  // the batched entry points run it: the two concurrent kernels (the subgroup test and
+  // the Miller loop) reach the dispatcher in another order from a graph than from two
+  // streams, and a persistent grid whose SIMDs the test's wavefronts took first starts
+  // part of its workgroups late (a 12-lane Miller loop at five workgroups per CU timed
+  // 20.3 ms from the streams and ran 22.0 ms from the graph; the 32-lane one 20.3 ms both
+  // ways).
   auto time_phase = [&](int which, double* ms, double bound) -> int {
+    struct Exec {
+      hipGraphExec_t e = nullptr;
+      ~Exec() {
+        if (e) (void)hipGraphExecDestroy(e);
+      }
+    } ex;
+    if (which == 2 && form == 0 && g.use_graph) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal));
+      const int rc = bls::engine_compressed(a, g.s);
+      const hipError_t ec = hipStreamEndCapture(g.s, &graph);
+      if (rc != 0 || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        g_err = "graph capture of a tuning candidate failed";
+        return -1;
+      }
+      const hipError_t ei = hipGraphInstantiate(&ex.e, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(ei);
+    }
     float t[3];
     int n = 0;
     for (int r = 0; r < 4; r++) {
       HIPCHK(hipEventRecord(e0, g.s));
-      CHK(which == 1 ? bls::engine_final_exp(a, g.s) : bls::engine_compressed(a, g.s));
+      if (ex.e) HIPCHK(hipGraphLaunch(ex.e, g.s));
+      else CHK(which == 1 ? bls::engine_final_exp(a, g.s) : bls::engine_compressed(a, g.s));
       HIPCHK(hipEventRecord(e1, g.s));
       HIPCHK(hipEventSynchronize(e1));
       if (r == 0) continue;

@@ -352,11 +352,13 @@ __device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
 // G1, DEC_NOT_IN_SUBGROUP otherwise.  [x^2]P is computed as [|x|]([|x|]P) without
 // leaving Jacobian coordinates, and compared with (beta x, -y) projectively:
 // X == beta x Z^2 and Y == -y Z^3 (the test of g1_in_subgroup).
-// The field products go through FpCalls: the test is latency work on few wavefronts,
-// and one copy of the multiplier keeps its loop small.
+// The field products go through FpCalls by default: beside the pairing kernels (the
+// riders of a pairing launch) one copy of the multiplier keeps the loop small in the
+// instruction cache they share; the standalone one-lane kernel inlines them.
 // An item that did not decode tests the generator instead, its result ignored: every
 // lane of the wavefront runs the whole test (a wavefront with 8 or fewer active lanes
 // issues about three times slower per instruction on this GPU).
+template <class Mul = FpCalls>
 __device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32* status, int M, int i) {
   const bool decoded = status[i] == DEC_OK;
   Fp x = fp_const(G1_GEN_X_M), y = fp_const(G1_GEN_Y_M);
@@ -365,13 +367,12 @@ __device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32*
     y = ld_soa(pts, NL, M, i);
   }
   Jac<Fp> p1 = {x, y, fp_one()};
-  Jac<Fp> a = mul_absx_jac<FpCalls>(p1);
-  Jac<Fp> b = mul_absx_jac<FpCalls>(a);
+  Jac<Fp> a = mul_absx_jac<Mul>(p1);
+  Jac<Fp> b = mul_absx_jac<Mul>(a);
   bool in = false;
   if (!is_zero(b.z)) {
-    Fp z2 = FpCalls::sqr(b.z);
-    in = eq(b.x, FpCalls::mul(FpCalls::mul(x, fp_const(BETA_M)), z2)) &&
-         eq(b.y, neg(FpCalls::mul(y, FpCalls::mul(z2, b.z))));
+    Fp z2 = Mul::sqr(b.z);
+    in = eq(b.x, Mul::mul(Mul::mul(x, fp_const(BETA_M)), z2)) && eq(b.y, neg(Mul::mul(y, Mul::mul(z2, b.z))));
   }
   return in || !decoded ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }

@@ -2,7 +2,7 @@
 // concurrently with the Miller loop (which needs only the decoded coordinates):
 //   k_g1_decode   flags, x < p, y = sqrt(x^3 + 4) with the sign choice -> point + status
 //   k_g1_subgroup [x^2] P == (beta x, -y) on decoded points -> subgroup status (four
-//                 lanes per item)
+//                 lanes per item; k_g1_subgroup_lane: one, for large launches)
 // Semantics as ic_bls12_381 G1Affine::from_compressed; the reference GPU decoder
 // (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the decoding steps
 // without any of the validity checks.  Compiled with the field multiplier inlined.
@@ -77,6 +77,19 @@ __global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ p
   if (live && q == 0) sub_status[i] = in || !decoded ? DEC_OK : DEC_NOT_IN_SUBGROUP;
 }
 
+// The same test on one lane per point, for launches past one wavefront per SIMD in
+// quads (more than 16,384 points: the batched forms, where the test runs beside the
+// Miller loop and takes issue cycles from it).  No exchange and no repeated products:
+// about half the instructions per point of the quad, which is what counts once every
+// SIMD is busy.
+__global__ __launch_bounds__(64, 1) void k_g1_subgroup_lane(const i32* __restrict__ pts, const i32* __restrict__ status,
+                                                         i32* __restrict__ sub_status, int M) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < M;
+  const int32_t st = g1_subgroup_status<FieldInline>(pts, status, M, live ? t : M - 1);
+  if (live) sub_status[t] = st;
+}
+
 // status := sub_status where status is OK (for entry points that report decoding alone)
 __global__ void k_merge_status(i32* __restrict__ status, const i32* __restrict__ sub_status, int M) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,8 +137,12 @@ int launch_g1_decode(const uint8_t* in, int32_t* pts, int32_t* status, int M, hi
   hipLaunchKernelGGL(k_g1_decode, dim3((M + block - 1) / block), dim3(block), 0, s, in, pts, status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+// quads up to one wavefront per SIMD (16,384 points), one lane per point beyond
 int launch_g1_subgroup(const int32_t* pts, const int32_t* status, int32_t* sub_status, int M, hipStream_t s) {
-  hipLaunchKernelGGL(k_g1_subgroup, dim3((4 * (size_t)M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
+  if (M > 16384)
+    hipLaunchKernelGGL(k_g1_subgroup_lane, dim3((M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
+  else
+    hipLaunchKernelGGL(k_g1_subgroup, dim3((4 * (size_t)M + 63) / 64), dim3(64), 0, s, pts, status, sub_status, M);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_merge_status(int32_t* status, const int32_t* sub_status, int M, hipStream_t s) {
