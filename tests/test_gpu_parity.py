@@ -5,6 +5,7 @@ import pytest
 
 import blsgpu
 import oracle as O
+from gpu_workload import make_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -95,6 +96,34 @@ def test_decompression_matches_oracle():
     assert (o == out[:24]).all()
     o2 = np.zeros((len(pts), 12), np.uint64)
     assert lib.gpu_g1_decompress_batch(O.ptr(np.ascontiguousarray(comp)), O.ptr(o2), len(pts)) == -1
+
+
+@pytest.mark.parametrize("n", [16384, 16385])
+def test_subgroup_test_at_its_layout_switch(n):
+    """The decompression entry point at the batch size where the subgroup test changes
+    layout (four lanes per point up to 16,384 points, one lane per point beyond): every
+    status against the categories the device generator planted (no square root, off the
+    subgroup; wrong-message signatures are valid points), with off-subgroup encodings also
+    first and last in the batch, the coordinates equal to the unchecked decoding's, and a
+    sample against the oracle."""
+    print("case: subgroup layouts, n=%d" % n, flush=True)
+    w = make_workload(17, 3, n, 32, invalid_every=61)
+    comp, cat = w["sig"].copy(), w["cat"].copy()
+    off = int(np.nonzero(cat == 3)[0][0])
+    for j in (0, n - 1):
+        comp[j], cat[j] = comp[off], 3
+    want = np.select([cat == 2, cat == 3], [-3, -4], 0)
+    out, st = blsgpu.decompress_status(comp)
+    assert (st == want).all(), np.nonzero(st != want)
+    out_u, st_u = blsgpu.decompress_status(comp, check_subgroup=False)
+    assert (st_u == np.where(cat == 2, -3, 0)).all()
+    ok = st == 0
+    assert (out[ok] == out_u[ok]).all()
+    for i in sorted(set(list(range(0, n, 1021)) + [off, n - 2, n - 1])):
+        rc, q = O.g1_decompress(comp[i])
+        assert st[i] == rc, i
+        if rc == 0:
+            assert (out[i] == q).all(), i
 
 
 def test_fused_compressed_path_matches_oracle():
