@@ -108,26 +108,6 @@ __device__ __forceinline__ Jac<F> smul(const Aff<F>& q, const u32 k[8], const F&
   return acc;
 }
 
-// [|x|] q for the BLS parameter |x| = 0xd201000000010000
-__device__ __forceinline__ Jac<Fp> g1_mul_absx(const Aff<Fp>& q) {
-  Jac<Fp> acc = {q.x, q.y, fp_one()};
-  for (int i = 62; i >= 0; i--) {
-    acc = jdbl(acc);
-    if ((BLS_X >> i) & 1) acc = jadd_aff(acc, q, fp_one());
-  }
-  return acc;
-}
-// G1 membership for a point on the curve: phi(P) = [-x^2]P with phi(x, y) = (beta x, y)
-// (eprint 2021/1130 sec. 6; ic_bls12_381's G1Affine::is_torsion_free).  Equivalently
-// [x^2]P == (beta x, -y).
-__device__ __forceinline__ bool g1_in_subgroup(const Aff<Fp>& p) {
-  Jac<Fp> a = g1_mul_absx(p);
-  if (is_zero(a.z)) return false;
-  Aff<Fp> a1 = to_affine(a);
-  Jac<Fp> b = g1_mul_absx(a1);
-  Aff<Fp> phi = {mul(p.x, fp_const(BETA_M)), neg(p.y)};
-  return jeq_aff(b, phi);
-}
 
 // ---- G2: membership and decompression (public keys) ----
 // Square root in Fp2 by the norm (p = 3 mod 4; eprint 2012/685, Algorithm 8, the
@@ -351,7 +331,9 @@ __device__ __forceinline__ Fp ld_soa(const i32* base, int w0, int M, int i) {
 // 14..27): DEC_OK for items that did not decode (their status stands) and for points of
 // G1, DEC_NOT_IN_SUBGROUP otherwise.  [x^2]P is computed as [|x|]([|x|]P) without
 // leaving Jacobian coordinates, and compared with (beta x, -y) projectively:
-// X == beta x Z^2 and Y == -y Z^3 (the test of g1_in_subgroup).
+// X == beta x Z^2 and Y == -y Z^3.  G1 membership of a point on the curve is
+// phi(P) == [-x^2]P with phi(x, y) = (beta x, y) (eprint 2021/1130 sec. 6; ic_bls12_381's
+// G1Affine::is_torsion_free).
 // The field products go through FpCalls by default: beside the pairing kernels (the
 // riders of a pairing launch) one copy of the multiplier keeps the loop small in the
 // instruction cache they share; the standalone one-lane kernel inlines them.
@@ -380,8 +362,9 @@ __device__ __forceinline__ int32_t g1_subgroup_status(const i32* pts, const i32*
 // 0..3).  Semantics of ic_bls12_381 G1Affine::from_compressed: the reference GPU kernel
 // (ic-integration/gpu_crypto/cuda/decompress.cu:140-188) performs the flag handling, the
 // x^3+4 square root by (p+1)/4 and the sign choice, but none of the validity checks,
-// which this routine adds.  check_subgroup=false gives from_compressed_unchecked.
-__device__ __forceinline__ int32_t g1_decompress(const u32 be[12], Aff<Fp>& out, bool check_subgroup) {
+// which this routine adds except the subgroup test: this is from_compressed_unchecked,
+// and the test runs apart on the decoded points (g1_subgroup_status, k_g1_subgroup).
+__device__ __forceinline__ int32_t g1_decompress(const u32 be[12], Aff<Fp>& out) {
   const u32 top = be[0] >> 24;
   const bool cflag = top & 0x80, iflag = top & 0x40, sflag = top & 0x20;
   u32 w[12];  // little-endian 32-bit words of x
@@ -414,7 +397,6 @@ __device__ __forceinline__ int32_t g1_decompress(const u32 be[12], Aff<Fp>& out,
   if (lex_largest(y) != sflag) y = neg(y);
   out.x = x;
   out.y = y;
-  if (check_subgroup && !g1_in_subgroup(out)) return DEC_NOT_IN_SUBGROUP;
   return DEC_OK;
 }
 // 48-byte compressed encoding of an affine point (big-endian words as above)

@@ -63,25 +63,6 @@ struct LinesAoS {  // one key table (same for every lane or chosen per lane)
 };
 // ---- kernels ----
 
-// compressed G1 (M x 48 bytes) -> internal SoA points + status per item
-__global__ __launch_bounds__(64, 1) void k_g1_decompress(const uint8_t* __restrict__ in, i32* __restrict__ pts, i32* __restrict__ status,
-                                int M, int check_subgroup) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = t < M;
-  const int i = live ? t : M - 1;
-  u32 be[12];
-  const uint8_t* b = in + (size_t)i * 48;
-#pragma unroll
-  for (int j = 0; j < 12; j++)
-    be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
-  Aff<Fp> P;
-  int32_t st = g1_decompress(be, P, check_subgroup != 0);
-  if (!live) return;
-  st_fp_soa(pts, 0, M, i, P.x);
-  st_fp_soa(pts, NL, M, i, P.y);
-  status[i] = st;
-}
-
 // internal SoA points -> reference form x | y (M x 12 u64); identity status -> zeros
 __global__ void k_points_to_ref(const i32* __restrict__ pts, const i32* __restrict__ status, u64* __restrict__ out,
                                 int M) {
@@ -169,11 +150,6 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
 int launch_table_to_ref(const int32_t* t, uint64_t* out, hipStream_t s) {
   hipLaunchKernelGGL(k_table_to_ref, dim3(nblk(NUM_COEFFS * 6, 64)), dim3(64), 0, s, t, out);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-int launch_g1_decompress(const uint8_t* in, int32_t* pts, int32_t* status, int M, int check_subgroup,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_g1_decompress, dim3(nblk(M, 64)), dim3(64), 0, s, in, pts, status, M, check_subgroup);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* out, int M, hipStream_t s) {

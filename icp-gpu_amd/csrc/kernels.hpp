@@ -23,8 +23,6 @@ int launch_hash_to_g1(const uint8_t* msgs, const int64_t* offs, int n, const uin
 // K compressed G2 points (96 bytes each) -> reference-form affine points + decode status
 int launch_g2_decompress(const uint8_t* in, uint64_t* q_ref, int32_t* status, int K, int check_subgroup,
                          hipStream_t s);
-int launch_g1_decompress(const uint8_t* in, int32_t* pts, int32_t* status, int M, int check_subgroup,
-                         hipStream_t s);
 int launch_points_to_ref(const int32_t* pts, const int32_t* status, uint64_t* out, int M, hipStream_t s);
 int launch_miller(const int32_t* sig, const int32_t* nhm, const int32_t* g2lines, const int32_t* pklines,
                   const int32_t* key_idx, int nkeys, int32_t* sig_status, int32_t* fout, int M, int block,

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(DECODE_BLOCK, 1) void k_g1_decode(const uint8_t* __
   for (int j = 0; j < 12; j++)
     be[j] = ((u32)b[4 * j] << 24) | ((u32)b[4 * j + 1] << 16) | ((u32)b[4 * j + 2] << 8) | b[4 * j + 3];
   Aff<Fp> P;
-  const int32_t st = g1_decompress(be, P, false);
+  const int32_t st = g1_decompress(be, P);
   if (!live) return;
   st_soa(pts, 0, M, i, P.x);
   st_soa(pts, NL, M, i, P.y);
