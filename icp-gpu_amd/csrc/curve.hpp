@@ -38,12 +38,6 @@ struct FpCalls {
   static __device__ __forceinline__ Fp mul(const Fp& a, const Fp& b) { return mul_call(a, b); }
   static __device__ __forceinline__ Fp sqr(const Fp& a) { return sqr_call(a); }
 };
-static __device__ __noinline__ Fp2 mul2_call(Fp2 a, Fp2 b) { return mul(a, b); }
-static __device__ __noinline__ Fp2 sqr2_call(Fp2 a) { return sqr(a); }
-struct Fp2Calls {
-  static __device__ __forceinline__ Fp2 mul(const Fp2& a, const Fp2& b) { return mul2_call(a, b); }
-  static __device__ __forceinline__ Fp2 sqr(const Fp2& a) { return sqr2_call(a); }
-};
 
 // dbl-2009-l (a = 0).  Z = 0 (identity) maps to Z3 = 0.
 template <class F, class M = FieldInline>
@@ -83,12 +77,6 @@ __device__ __forceinline__ Jac<F> jadd_aff(const Jac<F>& p, const Aff<F>& q, con
   r.y = norm(sub(M::mul(R, sub(V, r.x)), dbl(M::mul(p.y, J))));
   r.z = norm(sub(sub(M::sqr(carry(add(p.z, H))), Z1Z1), HH));
   return r;
-}
-template <class F, class M = FieldInline>
-__device__ __forceinline__ bool jeq_aff(const Jac<F>& p, const Aff<F>& q) {
-  if (is_zero(p.z)) return false;
-  F z2 = M::sqr(p.z);
-  return eq(p.x, M::mul(q.x, z2)) && eq(p.y, M::mul(q.y, M::mul(z2, p.z)));
 }
 template <class F>
 __device__ __forceinline__ Aff<F> to_affine(const Jac<F>& p) {
@@ -144,18 +132,6 @@ __device__ __forceinline__ bool lex_largest2(const Fp2& a) {
 __device__ __forceinline__ Aff<Fp2> g2_psi(const Aff<Fp2>& q) {
   const Fp2 cx = {fp_const(PSI_X0), fp_const(PSI_X1)}, cy = {fp_const(PSI_Y0), fp_const(PSI_Y1)};
   return {mul(conj(q.x), cx), mul(conj(q.y), cy)};
-}
-// G2 membership of a point on the twist: psi(Q) == [x]Q = -[|x|]Q (eprint 2021/1130
-// sec. 4, eprint 2022/352; ic_bls12_381's G2Affine::is_torsion_free)
-__device__ __forceinline__ bool g2_in_subgroup(const Aff<Fp2>& q) {
-  const Fp2 one = {fp_one(), fp_zero()};
-  Jac<Fp2> acc = {q.x, q.y, one};
-  for (int i = 62; i >= 0; i--) {
-    acc = jdbl<Fp2, Fp2Calls>(acc);
-    if ((BLS_X >> i) & 1) acc = jadd_aff<Fp2, Fp2Calls>(acc, q, one);
-  }
-  acc.y = neg(acc.y);
-  return jeq_aff<Fp2, Fp2Calls>(acc, g2_psi(q));
 }
 
 // ---- G1 decompression (ZCash encoding: bit7 compressed, bit6 infinity, bit5 sort) ----
@@ -399,12 +375,11 @@ __device__ __forceinline__ int32_t g1_decompress(const u32 be[12], Aff<Fp>& out)
   out.y = y;
   return DEC_OK;
 }
-// 48-byte compressed encoding of an affine point (big-endian words as above)
 // 96 bytes given as 24 big-endian-packed words: x.c1 (words 0..11, flags in the top
-// three bits) then x.c0.  Semantics of ic_bls12_381 G2Affine::from_compressed (the
-// reference has no G2 decompression; its callers pass prepared lines):
-// check_subgroup=false gives from_compressed_unchecked.
-__device__ __forceinline__ int32_t g2_decompress(const u32 be[24], Aff<Fp2>& out, bool check_subgroup) {
+// three bits) then x.c0.  Semantics of ic_bls12_381 G2Affine::from_compressed_unchecked
+// (the reference has no G2 decompression; its callers pass prepared lines); the
+// subgroup test (g2_subgroup_quad) runs apart.
+__device__ __forceinline__ int32_t g2_decompress(const u32 be[24], Aff<Fp2>& out) {
   const u32 top = be[0] >> 24;
   const bool cflag = top & 0x80, iflag = top & 0x40, sflag = top & 0x20;
   u32 w1[12], w0[12];  // little-endian 32-bit words of x.c1 and x.c0
@@ -444,10 +419,10 @@ __device__ __forceinline__ int32_t g2_decompress(const u32 be[24], Aff<Fp2>& out
   if (lex_largest2(y) != sflag) y = neg(y);
   out.x = x;
   out.y = norm(y);
-  if (check_subgroup && !g2_in_subgroup(out)) return DEC_NOT_IN_SUBGROUP;
   return DEC_OK;
 }
 
+// 48-byte compressed encoding of an affine point (big-endian words as above)
 __device__ __forceinline__ void g1_compress(const Aff<Fp>& p, bool identity, u32 be[12]) {
   if (identity) {
 #pragma unroll
@@ -466,62 +441,160 @@ __device__ __forceinline__ void g1_compress(const Aff<Fp>& p, bool identity, u32
 // ---- G2Prepared line coefficients (cuda/test_miller_v2.cu:152-228, eprint 2010/354
 // Alg. 26 and 27).  The projective representation of R is not canonical, so the
 // coefficient values depend on this exact formula sequence; it is the one that
-// reproduces cuda/g2_coeffs.h and cuda/pk42_coeffs.h. ----
+// reproduces cuda/g2_coeffs.h and cuda/pk42_coeffs.h.  On a lane quad (four lanes
+// holding the same key): each step's independent Fp2 products are split four ways, the
+// steps' values read across the quad by DPP, so a doubling step is three product
+// latencies (eleven products) and an addition step five (sixteen); every lane then holds
+// the whole line and the next R. ----
 struct G2Proj {
   Fp2 x, y, z;
 };
-__device__ __forceinline__ Line doubling_step(G2Proj& r) {
-  Fp2 tmp0 = sqr(r.x);
-  Fp2 tmp1 = sqr(r.y);
-  Fp2 tmp2 = sqr(tmp1);
-  Fp2 tmp3 = norm(dbl(sub(sqr(carry(add(tmp1, r.x))), add(tmp0, tmp2))));
-  Fp2 tmp4 = norm(add(dbl(tmp0), tmp0));
-  Fp2 tmp6 = carry(add(r.x, tmp4));
-  Fp2 tmp5 = sqr(tmp4);
-  Fp2 zsq = sqr(r.z);
-  Fp2 nx = norm(sub(sub(tmp5, tmp3), tmp3));
-  Fp2 nz = norm(sub(sqr(carry(add(r.z, r.y))), add(tmp1, zsq)));
-  Fp2 ny = norm(sub(mul(sub(tmp3, nx), tmp4), norm(dbl(dbl(dbl(tmp2))))));
-  r.x = nx; r.y = ny; r.z = nz;
-  Fp2 c1 = norm(neg(dbl(mul(tmp4, zsq))));
-  Fp2 c2 = norm(sub(sub(sqr(tmp6), add(tmp0, tmp5)), dbl(dbl(tmp1))));
-  Fp2 c0 = norm(dbl(mul(r.z, zsq)));
+template <int K>
+__device__ __forceinline__ Fp2 quad_get2(const Fp2& a) {
+  return {quad_get<K>(a.c0), quad_get<K>(a.c1)};
+}
+__device__ __forceinline__ Fp2 pick2(bool c, const Fp2& a, const Fp2& b) { return {pick(c, a.c0, b.c0), pick(c, a.c1, b.c1)}; }
+__device__ __forceinline__ Fp2 pick4_2(int q, const Fp2& a0, const Fp2& a1, const Fp2& a2, const Fp2& a3) {
+  return {pick4(q, a0.c0, a1.c0, a2.c0, a3.c0), pick4(q, a0.c1, a1.c1, a2.c1, a3.c1)};
+}
+__device__ __forceinline__ Line doubling_step_quad(G2Proj& r, int q) {
+  // X^2 | Y^2 | Z^2 | (Z + Y)^2
+  const Fp2 s1 = sqr(pick4_2(q, r.x, r.y, r.z, carry(add(r.z, r.y))));
+  const Fp2 tmp0 = quad_get2<0>(s1), tmp1 = quad_get2<1>(s1), zsq = quad_get2<2>(s1), zy2 = quad_get2<3>(s1);
+  const Fp2 tmp4 = norm(add(dbl(tmp0), tmp0));
+  const Fp2 tmp6 = carry(add(r.x, tmp4));
+  // tmp1^2 | (tmp1 + X)^2 | tmp4^2 | tmp6^2
+  const Fp2 s2 = sqr(pick4_2(q, tmp1, carry(add(tmp1, r.x)), tmp4, tmp6));
+  const Fp2 tmp2 = quad_get2<0>(s2), t13 = quad_get2<1>(s2), tmp5 = quad_get2<2>(s2), t66 = quad_get2<3>(s2);
+  const Fp2 tmp3 = norm(dbl(sub(t13, add(tmp0, tmp2))));
+  const Fp2 nx = norm(sub(sub(tmp5, tmp3), tmp3));
+  const Fp2 nz = norm(sub(zy2, add(tmp1, zsq)));
+  const Fp2 c2 = norm(sub(sub(t66, add(tmp0, tmp5)), dbl(dbl(tmp1))));
+  // (tmp3 - nx) tmp4 | tmp4 zsq | nz zsq (lane 3 repeats lane 0)
+  const Fp2 d = sub(tmp3, nx);
+  const Fp2 s3 = mul(pick4_2(q, d, tmp4, nz, d), pick4_2(q, tmp4, zsq, zsq, tmp4));
+  r.x = nx;
+  r.y = norm(sub(quad_get2<0>(s3), norm(dbl(dbl(dbl(tmp2))))));
+  r.z = nz;
+  const Fp2 c1 = norm(neg(dbl(quad_get2<1>(s3))));
+  const Fp2 c0 = norm(dbl(quad_get2<2>(s3)));
   return {c0, c1, c2};
 }
-__device__ __forceinline__ Line addition_step(G2Proj& r, const Aff<Fp2>& q) {
-  Fp2 zsq = sqr(r.z);
-  Fp2 ysq = sqr(q.y);
-  Fp2 t0 = mul(zsq, q.x);
-  Fp2 t1 = mul(norm(sub(sqr(carry(add(q.y, r.z))), add(ysq, zsq))), zsq);
-  Fp2 t2 = norm(sub(t0, r.x));
-  Fp2 t3 = sqr(t2);
-  Fp2 t4 = norm(dbl(dbl(t3)));
-  Fp2 t5 = mul(t4, t2);
-  Fp2 t6 = norm(sub(sub(t1, r.y), r.y));
-  Fp2 t9 = mul(t6, q.x);
-  Fp2 t7 = mul(t4, r.x);
-  Fp2 nx = norm(sub(sub(sub(sqr(t6), t5), t7), t7));
-  Fp2 nz = norm(sub(sqr(carry(add(r.z, t2))), add(zsq, t3)));
-  Fp2 t10 = carry(add(q.y, nz));
-  Fp2 t8 = mul(sub(t7, nx), t6);
-  Fp2 ny = norm(sub(t8, dbl(mul(r.y, t5))));
-  r.x = nx; r.y = ny; r.z = nz;
-  t10 = norm(sub(sub(sqr(t10), ysq), sqr(nz)));
-  Fp2 c2 = norm(sub(dbl(t9), t10));
-  Fp2 c0 = norm(dbl(nz));
-  Fp2 c1 = norm(dbl(neg(t6)));
+__device__ __forceinline__ Line addition_step_quad(G2Proj& r, const Aff<Fp2>& qa, int q) {
+  const bool h = q & 1;
+  // Z^2 | qy^2 | (qy + Z)^2 (lane 3 repeats lane 0)
+  const Fp2 s1 = sqr(pick4_2(q, r.z, qa.y, carry(add(qa.y, r.z)), r.z));
+  const Fp2 zsq = quad_get2<0>(s1), ysq = quad_get2<1>(s1), yz2 = quad_get2<2>(s1);
+  // t0 = zsq qx | t1 = ((qy + Z)^2 - qy^2 - zsq) zsq (lanes 2, 3 repeat 0, 1)
+  const Fp2 s2 = mul(pick2(h, norm(sub(yz2, add(ysq, zsq))), zsq), pick2(h, zsq, qa.x));
+  const Fp2 t2 = norm(sub(quad_get2<0>(s2), r.x));
+  const Fp2 t6 = norm(sub(sub(quad_get2<1>(s2), r.y), r.y));
+  // t3 = t2^2 | t9 = t6 qx | t6^2 | (Z + t2)^2
+  const Fp2 zt = carry(add(r.z, t2));
+  const Fp2 s3 = mul(pick4_2(q, t2, t6, t6, zt), pick4_2(q, t2, qa.x, t6, zt));
+  const Fp2 t3 = quad_get2<0>(s3), t9 = quad_get2<1>(s3), t66 = quad_get2<2>(s3), zt2 = quad_get2<3>(s3);
+  const Fp2 t4 = norm(dbl(dbl(t3)));
+  const Fp2 nz = norm(sub(zt2, add(zsq, t3)));
+  const Fp2 t10 = carry(add(qa.y, nz));
+  // t5 = t4 t2 | t7 = t4 X | t10^2 | nz^2
+  const Fp2 s4 = mul(pick4_2(q, t4, t4, t10, nz), pick4_2(q, t2, r.x, t10, nz));
+  const Fp2 t5 = quad_get2<0>(s4), t7 = quad_get2<1>(s4), t1010 = quad_get2<2>(s4), nz2 = quad_get2<3>(s4);
+  const Fp2 nx = norm(sub(sub(sub(t66, t5), t7), t7));
+  // t8 = (t7 - nx) t6 | Y t5 (lanes 2, 3 repeat 0, 1)
+  const Fp2 s5 = mul(pick2(h, r.y, sub(t7, nx)), pick2(h, t5, t6));
+  r.x = nx;
+  r.y = norm(sub(quad_get2<0>(s5), dbl(quad_get2<1>(s5))));
+  r.z = nz;
+  const Fp2 c2 = norm(sub(dbl(t9), norm(sub(sub(t1010, ysq), nz2))));
+  const Fp2 c0 = norm(dbl(nz));
+  const Fp2 c1 = norm(dbl(neg(t6)));
   return {c0, c1, c2};
 }
-// Emit the 68 lines of q in Miller-loop order through sink(k, line).
+// Emit the 68 lines of q in Miller-loop order through sink(k, line), on the quad.
 template <class Sink>
-__device__ __forceinline__ void g2_prepare(const Aff<Fp2>& q, Sink sink) {
-  G2Proj r = {q.x, q.y, fp2_one()};
+__device__ __forceinline__ void g2_prepare_quad(const Aff<Fp2>& qa, int q, Sink sink) {
+  G2Proj r = {qa.x, qa.y, fp2_one()};
   int n = 0;
+#pragma unroll 1
   for (int b = 61; b >= 0; b--) {
-    sink(n++, doubling_step(r));
-    if (miller_bit(b)) sink(n++, addition_step(r, q));
+    sink(n++, doubling_step_quad(r, q));
+    if (miller_bit(b)) sink(n++, addition_step_quad(r, qa, q));
   }
-  sink(n++, doubling_step(r));
+  sink(n++, doubling_step_quad(r, q));
+}
+
+// ---- G2 membership on the quad: psi(Q) == [x]Q = -[|x|]Q (eprint 2021/1130 sec. 4,
+// eprint 2022/352; ic_bls12_381's G2Affine::is_torsion_free).  [|x|]Q by the complete
+// projective formulas of quad_dbl / quad_add over Fp2 for the twist y^2 = x^3 + 4(1 + u)
+// (b3 = 12(1 + u), so the b3 terms become integer combinations of an Fp2 value's two
+// components), then X == psi_x Z and -Y == psi_y Z with Z != 0. ----
+struct Prj2 {
+  Fp2 x, y, z;
+};
+// c0 a0 + c1 a1 + c2 a2 + c3 a3 for N1 values, |c| <= 128 (as lin3): N1 out
+__device__ __forceinline__ Fp lin4(const Fp& a0, int c0, const Fp& a1, int c1, const Fp& a2, int c2, const Fp& a3,
+                                   int c3) {
+  Fp r;
+  i64 acc = 0;
+#pragma unroll
+  for (int i = 0; i < NL - 1; i++) {
+    acc += (i64)c0 * a0.l[i] + (i64)c1 * a1.l[i] + (i64)c2 * a2.l[i] + (i64)c3 * a3.l[i];
+    r.l[i] = (i32)acc & MASK28;
+    acc >>= 28;
+  }
+  r.l[NL - 1] = (i32)(acc + (i64)c0 * a0.l[NL - 1] + (i64)c1 * a1.l[NL - 1] + (i64)c2 * a2.l[NL - 1] +
+                      (i64)c3 * a3.l[NL - 1]);
+  return fold_top(r);
+}
+// k0 a + k1 (1 + u) b + k2 c for Fp2 values with N1 components ((1 + u) b =
+// (b0 - b1) + (b0 + b1) u): N1 components out
+__device__ __forceinline__ Fp2 lin_xi(const Fp2& a, int k0, const Fp2& b, int k1, const Fp2& c, int k2) {
+  return {lin4(a.c0, k0, b.c0, k1, b.c1, -k1, c.c0, k2), lin4(a.c1, k0, b.c0, k1, b.c1, k1, c.c1, k2)};
+}
+__device__ __forceinline__ Fp2 lin3_2(const Fp2& a0, int c0, const Fp2& a1, int c1, const Fp2& a2, int c2) {
+  return {lin3(a0.c0, c0, a1.c0, c1, a2.c0, c2), lin3(a0.c1, c0, a1.c1, c1, a2.c1, c2)};
+}
+// 2Q (as quad_dbl; Y out N2)
+__device__ __forceinline__ Prj2 quad_dbl2(const Prj2& p, int q) {
+  const Fp2 s1 = mul(pick2(q == 3, p.x, pick2(q == 2, p.z, p.y)), pick2(q == 1 || q == 2, p.z, p.y));
+  const Fp2 y2 = quad_get2<0>(s1), yz = quad_get2<1>(s1), z2 = quad_get2<2>(s1), xy = quad_get2<3>(s1);
+  // c0 Y^2 + c2 (1 + u) Z^2 + c1 YZ and d0 Y^2 + d1 (1 + u) Z^2 + d2 XY
+  const int c0 = q >= 2 ? 1 : 0, c1 = q == 1 ? 8 : 0, c2 = q == 0 ? 96 : (q >= 2 ? -36 : 0);
+  const int d0 = q == 3 ? 0 : 1, d1 = q == 2 ? 12 : 0, d2 = q == 3 ? 2 : 0;
+  const Fp2 s2 = mul(lin_xi(y2, c0, z2, c2, yz, c1), lin_xi(y2, d0, z2, d1, xy, d2));
+  return {quad_get2<3>(s2), add(quad_get2<0>(s2), quad_get2<2>(s2)), quad_get2<1>(s2)};
+}
+// P + R (as quad_add; N1 components in and out)
+__device__ __forceinline__ Prj2 quad_add2(const Prj2& p, const Prj2& r, int q) {
+  const bool h = q & 1;
+  const Fp2 s1 = mul(pick4_2(q, p.x, p.y, p.z, add(p.x, p.y)), pick4_2(q, r.x, r.y, r.z, add(r.x, r.y)));
+  const Fp2 s1b = mul(pick2(h, add(p.x, p.z), add(p.y, p.z)), pick2(h, add(r.x, r.z), add(r.y, r.z)));
+  const Fp2 t0 = quad_get2<0>(s1), t1 = quad_get2<1>(s1), t2 = quad_get2<2>(s1), t3 = quad_get2<3>(s1);
+  const Fp2 t4 = quad_get2<0>(s1b), t5 = quad_get2<1>(s1b);
+  const Fp2 xy = lin3_2(t3, 1, t0, -1, t1, -1), yz = lin3_2(t4, 1, t1, -1, t2, -1);
+  const Fp2 xz3 = lin_xi(t5, 0, lin3_2(t5, 1, t0, -1, t2, -1), 12, t5, 0);  // b3 xz
+  const Fp2 ym = lin_xi(t1, 1, t2, -12, t2, 0), yp = lin_xi(t1, 1, t2, 12, t2, 0);
+  const Fp2 x3 = lin3_2(t0, 3, t0, 0, t0, 0);
+  const Fp2 s2 = mul(pick4_2(q, yz, xy, xz3, ym), pick4_2(q, xz3, ym, x3, yp));
+  const Fp2 s2b = mul(pick2(h, yp, x3), pick2(h, yz, xy));
+  const Fp2 A = quad_get2<0>(s2), B = quad_get2<1>(s2), C = quad_get2<2>(s2), D = quad_get2<3>(s2);
+  const Fp2 E = quad_get2<0>(s2b), F = quad_get2<1>(s2b);
+  return {norm(sub(B, A)), norm(add(D, C)), norm(add(F, E))};
+}
+// whether a point on the twist lies in G2 (every lane of the quad, same answer)
+__device__ __forceinline__ bool g2_subgroup_quad(const Aff<Fp2>& qa, int q) {
+  const Prj2 p = {qa.x, qa.y, fp2_one()};
+  Prj2 acc = p;
+#pragma unroll 1
+  for (int i = 62; i >= 0; i--) {
+    acc = quad_dbl2(acc, q);
+    if ((BLS_X >> i) & 1) {
+      acc.y = norm(acc.y);
+      acc = quad_add2(acc, p, q);
+    }
+  }
+  const Aff<Fp2> ps = g2_psi(qa);
+  return !is_zero(acc.z) && eq(acc.x, mul(ps.x, acc.z)) && eq(neg(acc.y), mul(ps.y, acc.z));
 }
 
 }  // namespace bls
