@@ -80,8 +80,8 @@ __global__ __launch_bounds__(64, 1) void k_g1_subgroup(const i32* __restrict__ p
 // The same test on one lane per point, for launches past one wavefront per SIMD in
 // quads (more than 16,384 points: the batched forms, where the test runs beside the
 // Miller loop and takes issue cycles from it).  No exchange and no repeated products:
-// about half the instructions per point of the quad, which is what counts once every
-// SIMD is busy.
+// two thirds of the quad's instructions per point (9.9 k against 14.8 k VALU), which is
+// what counts once every SIMD is busy.
 __global__ __launch_bounds__(64, 1) void k_g1_subgroup_lane(const i32* __restrict__ pts, const i32* __restrict__ status,
                                                          i32* __restrict__ sub_status, int M) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
