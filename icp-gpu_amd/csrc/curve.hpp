@@ -99,14 +99,17 @@ __device__ __forceinline__ Jac<F> smul(const Aff<F>& q, const u32 k[8], const F&
 
 // ---- G2: membership and decompression (public keys) ----
 // Square root in Fp2 by the norm (p = 3 mod 4; eprint 2012/685, Algorithm 8, the
-// "complex method"), with two square roots in Fp and one inversion where two
-// exponentiations in Fp2 took 2.6 times the products.  For a = a0 + a1 u:
+// "complex method"), with two exponentiations in Fp where two exponentiations in Fp2
+// took 2.6 times the products.  For a = a0 + a1 u:
 //   lambda = sqrt(a0^2 + a1^2) (a is a square in Fp2 exactly when its norm is one in Fp);
 //   delta = (a0 + lambda) / 2 and t = delta^((p+1)/4).  t^2 = delta gives the root
 //   (t, a1 / (2t)); t^2 = -delta gives (a1 / (2t), t): the product of the two candidate
 //   deltas (a0 +- lambda) / 2 is -(a1/2)^2, so exactly one is a square (-1 is not one) and
 //   a1 / (2t) is the root of the other.  a1 = 0 takes delta = a0 and a zero partner
 //   coordinate: (t, 0) when a0 is a square, else (0, t).
+// The second exponentiation is s = delta^((p-3)/4), the same length: t = delta s, and
+// t s = delta^((p-1)/2) = +-1, so 1/t = s (t s) needs no inversion (a zero delta gives
+// zero, as the inversion gave; it occurs with a1 != 0 only for a non-square a).
 // ok = the result squares back to a.  Which of the two roots comes out does not matter:
 // the caller picks the sign by the sort flag.
 __device__ __forceinline__ Fp2 sqrt2(const Fp2& a, bool& ok) {
@@ -114,10 +117,11 @@ __device__ __forceinline__ Fp2 sqrt2(const Fp2& a, bool& ok) {
   const Fp n = norm(add(sqr(a.c0), sqr(a.c1)));
   const Fp lambda = sqrt_cand(n);
   const Fp delta = a1z ? a.c0 : mul(norm(add(a.c0, lambda)), fp_const(HALF_M));
-  const Fp t = sqrt_cand(delta);
+  const Fp s = pow_win(delta, P34_WIN);
+  const Fp t = mul(delta, s);
   const bool sq = eq(sqr(t), delta);
   Fp w = fp_zero();
-  if (!a1z) w = mul(a.c1, inv_fp(dbl(t)));  // t != 0: delta = 0 only when a1 = 0
+  if (!a1z) w = mul(mul(a.c1, mul(s, mul(t, s))), fp_const(HALF_M));  // a1 / (2t)
   Fp2 r = sq ? Fp2{t, w} : Fp2{w, t};
   r = norm(r);
   ok = eq(sqr(r), a);
