@@ -60,6 +60,18 @@ def test_matches_oracle_over_lengths(dst):
         assert _affine(neg[i]) == (x, (-y) % O.P), i
 
 
+@pytest.mark.parametrize("n", [1, 3, 17])
+def test_small_batches_against_oracle(n):
+    # one quad per message: a single message leaves 60 lanes of its wavefront repeating
+    # it, 3 and 17 end a wavefront part way through; every output against the oracle
+    print("case: hash_to_g1 n=%d" % n, flush=True)
+    msgs = _messages(90 + n, [(7 * i) % 70 for i in range(n)])
+    out, ident = blsgpu.hash_to_g1(msgs, h2c.IC_DST)
+    assert not ident.any()
+    for i, m in enumerate(msgs):
+        assert _affine(out[i]) == h2c.hash_to_g1(m, h2c.IC_DST), (n, i)
+
+
 def test_fused_wire_path():
     print("case: verify_messages M=40", flush=True)
     blsgpu.load().blsgpu_reset_keys()
