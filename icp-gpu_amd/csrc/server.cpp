@@ -83,6 +83,7 @@ struct blsgpu_server {
   double st_steps = 0, st_items = 0, st_completed = 0, st_max_step = 0, st_max_inflight = 0, st_failed = 0;
 
   int setup_slot(Slot& sl);
+  static void release_slot(Slot& sl);
   int launch(Slot& sl);
   void finish(Slot& sl, bool ok);
   void loop();
@@ -112,6 +113,23 @@ int blsgpu_server::setup_slot(Slot& sl) {
   CHK(sl.h_res.ensure(m * 4));
   if (want_gt) CHK(sl.h_gt.ensure(m * 576));
   return 0;
+}
+
+// Frees what setup_slot made of the slot, once its streams are idle (also for a slot
+// whose setup stopped part way).
+void blsgpu_server::release_slot(Slot& sl) {
+  if (sl.s) (void)hipStreamSynchronize(sl.s);
+  if (sl.s2) (void)hipStreamSynchronize(sl.s2);
+  for (DevBuf* b : {&sl.comp, &sl.nhm, &sl.key, &sl.sig_pts, &sl.nhm_pts, &sl.status, &sl.sub, &sl.fbuf, &sl.res, &sl.gt,
+                    &sl.queue})
+    if (b->p) (void)hipFree(b->p);
+  for (PinnedBuf* b : {&sl.h_comp, &sl.h_nhm, &sl.h_key, &sl.h_res, &sl.h_gt})
+    if (b->p) (void)hipHostFree(b->p);
+  if (sl.s) (void)hipStreamDestroy(sl.s);
+  if (sl.s2) (void)hipStreamDestroy(sl.s2);
+  for (hipEvent_t e : {sl.fork, sl.join, sl.done})
+    if (e) (void)hipEventDestroy(e);
+  sl = Slot();
 }
 
 // Packs queued requests into the slot and launches its step (caller holds mu).
@@ -254,8 +272,12 @@ blsgpu_server* blsgpu_server_create(int slots, int max_step_items, int64_t hbm_b
     return nullptr;
   }
   srv->slots.resize(slots);
-  for (Slot& sl : srv->slots)
-    if (srv->setup_slot(sl) != 0) return nullptr;  // partially built slots are not freed on this path
+  for (Slot& sl : srv->slots) {
+    if (srv->setup_slot(sl) != 0) {
+      for (Slot& b : srv->slots) blsgpu_server::release_slot(b);
+      return nullptr;
+    }
+  }
   srv->th = std::thread([s = srv.get()] { s->loop(); });
   return srv.release();
 }
@@ -353,19 +375,7 @@ void blsgpu_server_destroy(blsgpu_server* srv) {
     srv->cv_work.notify_all();
   }
   if (srv->th.joinable()) srv->th.join();
-  for (Slot& sl : srv->slots) {
-    if (sl.s) (void)hipStreamSynchronize(sl.s);
-    if (sl.s2) (void)hipStreamSynchronize(sl.s2);
-    for (DevBuf* b : {&sl.comp, &sl.nhm, &sl.key, &sl.sig_pts, &sl.nhm_pts, &sl.status, &sl.sub, &sl.fbuf, &sl.res,
-                      &sl.gt, &sl.queue})
-      if (b->p) (void)hipFree(b->p);
-    for (PinnedBuf* b : {&sl.h_comp, &sl.h_nhm, &sl.h_key, &sl.h_res, &sl.h_gt})
-      if (b->p) (void)hipHostFree(b->p);
-    if (sl.s) (void)hipStreamDestroy(sl.s);
-    if (sl.s2) (void)hipStreamDestroy(sl.s2);
-    for (hipEvent_t e : {sl.fork, sl.join, sl.done})
-      if (e) (void)hipEventDestroy(e);
-  }
+  for (Slot& sl : srv->slots) blsgpu_server::release_slot(sl);
   delete srv;
 }
 

@@ -49,7 +49,8 @@ int blsgpu_server_wait(blsgpu_server* srv, int64_t id, int32_t* results, uint64_
 /* out[0] steps launched, [1] items launched, [2] requests completed, [3] largest step
  * (items), [4] most steps in flight at once, [5] step capacity, [6] slots, [7] failed steps */
 int blsgpu_server_stats(blsgpu_server* srv, double out[8]);
-/* Drops queued requests, waits for the steps on the GPU, frees everything. */
+/* Drops queued requests, waits for the steps on the GPU, frees everything.  No other
+ * call on srv may be in progress when it starts, or follow it. */
 void blsgpu_server_destroy(blsgpu_server* srv);
 
 /* Load generator for measurements: Poisson arrivals at items_per_s for `seconds`, request
