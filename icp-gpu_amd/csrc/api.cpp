@@ -138,13 +138,17 @@ int init_locked() {
     return -1;
   }
   HIPCHK(hipSetDevice(g.device));
-  HIPCHK(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&g.ev_h2c, hipEventDisableTiming));
-  for (hipEvent_t& e : g.ev_ring) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  for (hipEvent_t& e : g.ev_part) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // each object is made once: a set-up that failed part way resumes where it stopped
+  // on the next call instead of making (and leaking) the earlier objects again
+  auto stream = [](hipStream_t& st) { return st ? hipSuccess : hipStreamCreateWithFlags(&st, hipStreamNonBlocking); };
+  auto event = [](hipEvent_t& e) { return e ? hipSuccess : hipEventCreateWithFlags(&e, hipEventDisableTiming); };
+  HIPCHK(stream(g.s));
+  HIPCHK(stream(g.s2));
+  HIPCHK(event(g.ev_fork));
+  HIPCHK(event(g.ev_join));
+  HIPCHK(event(g.ev_h2c));
+  for (hipEvent_t& e : g.ev_ring) HIPCHK(event(e));
+  for (hipEvent_t& e : g.ev_part) HIPCHK(event(e));
   // the fixed workspace's device address on this device (g.device is current)
   void* sp = nullptr;
   HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -153,7 +157,7 @@ int init_locked() {
     return -1;
   }
   g.scratch = sp;
-  HIPCHK(hipEventCreateWithFlags(&g.ev_scratch, hipEventDisableTiming));
+  HIPCHK(event(g.ev_scratch));
   // generator lines, once
   CHK(g.g2_ref.ensure(24 * 8));
   CHK(g.gen_lines.ensure(bls::KEY_TABLE_WORDS * 4));
