@@ -77,6 +77,7 @@ int ensure_batched_ws(int M) {
   CHK(g.b_sub.ensure(m * 4));
   CHK(g.b_fbuf.ensure(m * bls::MILLER_WORDS * 4));
   CHK(g.b_queue.ensure(64));
+  CHK(g.b_park.ensure((size_t)bls::final_exp_park_words() * 4));
   return 0;
 }
 
@@ -105,6 +106,8 @@ bls::EngineArgs batched_args(const uint8_t* comp, const uint64_t* nhm, const int
   a.fork = g.ev_fork;
   a.join = g.ev_join;
   a.fbuf = g.b_fbuf.as<int32_t>();
+  a.park = g.b_park.p ? g.b_park.as<int32_t>() : nullptr;
+  a.park_items = bls::final_exp_park_words() / (6 * 168);
   a.results = res;
   a.gt = gt;
   return a;

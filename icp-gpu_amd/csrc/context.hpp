@@ -142,6 +142,7 @@ struct Ctx {
   // (ceil(log2 M) -> autotuned configuration) and the graph of the last call
   DevBuf b_comp, b_nhm, b_keyidx, b_res, b_gt;
   DevBuf b_sig_pts, b_nhm_pts, b_status, b_sub, b_fbuf, b_queue;
+  DevBuf b_park;  // homes of the compact final exponentiation (bls::final_exp_park_words())
   BatchedCfg bcfg;
   std::map<int, BatchedCfg> tuned;
   bool use_graph = true;

@@ -70,6 +70,12 @@ class Rec:
         self.products.append((a, b))
         return Lin({(T, len(self.products) - 1): 1})
 
+    def reserve(self, n):
+        """n T slots that an earlier op has filled: read as products, computed by no round"""
+        start = len(self.products)
+        self.products += [None] * n
+        return [Lin({(T, start + i): 1}) for i in range(n)]
+
 
 class ConstTable:
     def __init__(self):
@@ -504,6 +510,26 @@ def op_fl():
     return R, outs_of(f12mul(R, f12(A), ll_operand()))
 
 
+def op_mul_s():
+    # The Fp12 product in two ops, for kernels whose LDS holds fewer product slots:
+    # OP_MUL_S forms s = (a0 + a1)(b0 + b1) (an Fp6, 6 outputs); OP_MUL_PQ reads it from
+    # T slots 0..5, forms a0 b0 and a1 b1 in T slots 6.. and combines the three as
+    # f12mul does.
+    R = Rec()
+    a, b = f12(A), f12(B)
+    s = f6mul(R, a.c0 + a.c1, b.c0 + b.c1)
+    return R, [x for c in s.c for x in (c.c0, c.c1)]
+
+
+def op_mul_pq():
+    R = Rec()
+    t = R.reserve(6)
+    s = F6(F2(t[0], t[1]), F2(t[2], t[3]), F2(t[4], t[5]))
+    a, b = f12(A), f12(B)
+    aa, bb = f6mul(R, a.c0, b.c0), f6mul(R, a.c1, b.c1)
+    return R, outs_of(F12(aa + bb.mul_v(), s - aa - bb))
+
+
 OPS = [
     ("OP_MUL", op_mul),
     ("OP_SQR", op_sqr),
@@ -528,6 +554,8 @@ OPS = [
     ("OP_FL_SCALE", op_fl_scale),
     ("OP_FL", op_fl),
     ("OP_SQR_LL", op_sqr_ll),
+    ("OP_MUL_S", op_mul_s),
+    ("OP_MUL_PQ", op_mul_pq),
 ]
 
 
@@ -578,7 +606,7 @@ def op_programs(cap=None):
             R, outs = fn()
         finally:
             CAP_TERMS = None
-        for a, b in R.products:
+        for a, b in (p for p in R.products if p is not None):
             # operands: at most 8 terms and weight 16 (|value| < 2^386.4 keeps the
             # Montgomery product N1); outputs: weight < 2^12 (64-bit accumulation)
             assert len(a.t) <= MAX_OPERAND_TERMS and len(b.t) <= MAX_OPERAND_TERMS, name
@@ -623,8 +651,8 @@ def build_rounds_v(G, variants):
 def build_rounds_one(G, progs):
     ops = []
     for name, prods, outs, _ in progs:
-        order = sorted(range(len(prods)), key=lambda j: (len(prods[j][0].t) + len(prods[j][1].t),
-                                                         len(prods[j][0].t)))
+        order = sorted((j for j in range(len(prods)) if prods[j] is not None),
+                       key=lambda j: (len(prods[j][0].t) + len(prods[j][1].t), len(prods[j][0].t)))
         prounds = []
         for r0 in range(0, len(order), G):
             chunk = order[r0:r0 + G]
@@ -679,10 +707,15 @@ def limbs28(v):
 FE_STATIC_OPS = ("OP_MUL", "OP_CYC_SQR")
 ML_STATIC_OPS = ("OP_LL", "OP_FL_SCALE", "OP_SQR_LL")
 ML_STATIC_LANES = (12, 20, 24, 32)
+# the split product of the compact final exponentiation (12 lanes), emitted after the
+# others so that their records keep their places
+FEC_STATIC_OPS = ("OP_MUL_S", "OP_MUL_PQ")
+FEC_STATIC_LANES = (12,)
 
 
 def static_ops(G):
-    return FE_STATIC_OPS + (ML_STATIC_OPS if G in ML_STATIC_LANES else ())
+    return FE_STATIC_OPS + (ML_STATIC_OPS if G in ML_STATIC_LANES else ()) + \
+        (FEC_STATIC_OPS if G in FEC_STATIC_LANES else ())
 MIXED = 4
 
 
@@ -816,7 +849,8 @@ def main():
              "namespace sched {"]
     for k, (name, prods, outs, ro) in enumerate(progs):
         lines.append("constexpr int %s = %d;  // %d products, %d outputs%s" %
-                     (name, k, len(prods), sum(o is not None for o in outs), ", outputs read other inputs" if ro else ""))
+                     (name, k, sum(p is not None for p in prods), sum(o is not None for o in outs),
+                      ", outputs read other inputs" if ro else ""))
     lines.append("constexpr int NUM_OPS = %d;" % len(progs))
     max_t = max(len(prods) for G in GS for _, _, _, prods, _ in build_rounds_v(G, variants))
     lines.append("constexpr int MAX_PRODUCTS = %d;  // T slots an item needs (any op, any G)" % max_t)
@@ -855,7 +889,8 @@ def main():
     with open(os.path.join(HERE, "schedules_gen.hpp"), "w") as f:
         f.write("\n".join(lines) + "\n")
     for name, prods, outs, ro in progs:
-        print("%-12s products %3d outputs %3d%s" % (name, len(prods), sum(o is not None for o in outs),
+        print("%-12s products %3d outputs %3d%s" % (name, sum(p is not None for p in prods),
+                                                   sum(o is not None for o in outs),
                                                    "  (reads other inputs)" if ro else ""))
     for G, (nb, nr) in sizes.items():
         print("G=%2d: %4d rounds, %6d bytes of records" % (G, nr, nb))

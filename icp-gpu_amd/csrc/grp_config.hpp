@@ -66,13 +66,26 @@ int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, co
 // the depth grp_launch_ml runs for these arguments (M items)
 template <int G>
 int grp_ml_depth(bool queue, int wg_per_cu, bool riders, int depth, int M);
+// park / park_items: homes of the compact final exponentiation (12 lanes, queue-fed
+// launches of more workgroups per CU than the full-LDS kernel holds), for park_items
+// resident items; without them such a launch runs the full-LDS kernel at its occupancy
 template <int G>
 int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
-                  int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+                  int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, int32_t* park, long park_items,
+                  hipStream_t s);
 template <int G>
 int grp_occ_ml();
 template <int G>
 int grp_occ_fe();
+// resident workgroups per CU of the compact final exponentiation (0 where G has none)
+template <int G>
+int grp_occ_fe_compact();
+// the compact 12-lane final exponentiation, in a translation unit of its own
+// (kernels_grp.hip built with GRP_KIND 2): its occupancy, and a queue-fed launch of grid
+// workgroups whose items have homes in park (6 x 168 words per item)
+int fe_compact_occ();
+int fe_compact_launch(int grid, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
+                      uint64_t* gt, int M, int* queue, int32_t* park, hipStream_t s);
 
 // number of compute units of the current device (queried once)
 int grp_num_cus();

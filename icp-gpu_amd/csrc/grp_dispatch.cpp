@@ -37,9 +37,12 @@ int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth, i
   }
 }
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
+                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, int32_t* park,
+                         long park_items, hipStream_t s) {
   switch (G) {
-#define FE_CASE(g) case g: return grp_launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, riders, s);
+#define FE_CASE(g)                                                                                        \
+  case g:                                                                                                 \
+    return grp_launch_fe<g>(fin, status, sub_status, result, gt, M, queue, wg_per_cu, riders, park, park_items, s);
     BLS_GRP_FE_LANES(FE_CASE)
 #undef FE_CASE
     default: return -1;
@@ -112,12 +115,18 @@ int occupancy_grp(int which, int G) {
       default: return -1;
     }
   }
+  // the final exponentiation's compact form (12 lanes) fits more workgroups per CU
   switch (G) {
-#define FE_CASE(g) case g: return grp_occ_fe<g>();
+#define FE_CASE(g) case g: return grp_occ_fe_compact<g>() > grp_occ_fe<g>() ? grp_occ_fe_compact<g>() : grp_occ_fe<g>();
     BLS_GRP_FE_LANES(FE_CASE)
 #undef FE_CASE
     default: return -1;
   }
+}
+
+// homes of 6 x 168 words per resident item of the compact 12-lane final exponentiation
+long final_exp_park_words() {
+  return (long)grp_num_cus() * grp_occ_fe_compact<12>() * Cfg<12>::IPW * 6 * 168;
 }
 
 }  // namespace bls

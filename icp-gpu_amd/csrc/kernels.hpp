@@ -55,7 +55,11 @@ int launch_miller_grp(int G, const int32_t* sig, const int32_t* nhm, const int32
                       int* queue, int wg_per_cu, int depth, const SubgroupRiders& riders, hipStream_t s);
 int miller_grp_depth(int G, bool queue, int wg_per_cu, bool riders, int depth, int M);
 int launch_final_exp_grp(int G, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
-                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s);
+                         uint64_t* gt, int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, int32_t* park,
+                         long park_items, hipStream_t s);
+// int32 words of the homes a queue-fed final exponentiation may need for the values it
+// keeps out of LDS (its compact form), at any workgroups per CU; 0 if none
+long final_exp_park_words();
 // Items per subgroup-test workgroup that a one-workgroup-per-tile launch of M items at G
 // lanes per item (which: 0 Miller, 1 final exp) can carry on the SIMDs its tiles leave
 // without a wavefront; 0 if those SIMDs cannot hold the test of M items.

@@ -18,8 +18,9 @@
 // lines whichever XCD runs the workgroup.
 //
 // This file is compiled once per lane count and kernel (-DGRP_LANES=G -DGRP_KIND=0 for
-// the Miller loop, 1 for the final exponentiation); the launchers are instantiated at
-// the end, the dispatch over G is in grp_dispatch.cpp.
+// the Miller loop, 1 for the final exponentiation, 2 with GRP_LANES=12 for the compact
+// final exponentiation); the launchers are instantiated at the end, the dispatch over G
+// is in grp_dispatch.cpp.
 #include <hip/hip_runtime.h>
 
 #include "curve.hpp"
@@ -28,7 +29,7 @@
 #include "schedules_gen.hpp"
 
 #if !defined(GRP_LANES) || !defined(GRP_KIND)
-#error "kernels_grp.hip is compiled once per lane count and kernel: -DGRP_LANES=<G> -DGRP_KIND=<0|1>"
+#error "kernels_grp.hip is compiled once per lane count and kernel: -DGRP_LANES=<G> -DGRP_KIND=<0|1|2>"
 #endif
 
 namespace bls {
@@ -835,6 +836,229 @@ __global__ __launch_bounds__(Cfg<G>::THREADS, RIDE ? 1 : 2) void k_final_exp_grp
   }
 }
 
+// ---------------- final exponentiation in compact LDS ----------------
+// (a translation unit of its own, GRP_KIND 2: next to the full-LDS kernel its code moved
+// that kernel's code and made it 8 % slower, 1.45 -> 1.57 ms per 4,096 items)
+#if GRP_KIND == 2
+// The same exponentiation with 66 slots per item instead of 134, so that twice as many
+// workgroups fit a CU (two wavefronts per SIMD at 12 lanes: 8 workgroups of 19.5 KB).
+// Per item: two Fp12 areas X | Y (0..11, 12..23) and a scratch region S (24..65) that
+// holds the products of an op, or, in an Fp12 product, the first half s = (a0 + a1)
+// (b0 + b1) in its first 6 slots (OP_MUL_S) and the products of OP_MUL_PQ after them.
+// The values of the hard part that the current op does not touch wait in device memory:
+// each item has six homes of 168 words in PARK (its workgroup's slot of the persistent
+// grid, blockIdx.x * IPW + item), and lane s of the item moves component s, so every
+// word goes back to the lane that stored it.
+constexpr int FEC_SLOTS = 66;
+constexpr int FEC_X = 0, FEC_Y = 12 * SLOT, FEC_S = 24 * SLOT;
+constexpr int FEC_HOMES = 6, FEC_HOME_WORDS = 12 * NL;  // homes per item, words per home
+enum { H_F = 0, H_T0, H_T1, H_T2, H_T3, H_T4 };
+
+template <int G>
+constexpr size_t final_exp_compact_lds_bytes() {
+  return ((size_t)(Cfg<G>::IPW * FEC_SLOTS + sched::NUM_CONSTS) * SLOT + Cfg<G>::IPW * 12) * 4;
+}
+
+// a *= b, the product split in two ops through S: straight-line (in the cyclotomic
+// powers) or through the interpreter (the glue between them, run once per tile)
+template <int G>
+__device__ __forceinline__ void mul_compact(int sub, bool act, int a, int b, int S, int K) {
+  run_op_c<G, sched::OP_MUL_S>(sub, act, a, b, S, S + 6 * SLOT, K);
+  run_op_c<G, sched::OP_MUL_PQ>(sub, act, a, b, a, S, K);
+}
+template <int G, int WPE>
+__device__ __forceinline__ void mul_compact_i(int sub, bool act, int a, int b, int S, int K) {
+  run_op<G, WPE>(sched::OP_MUL_S, sub, act, a, b, S, S + 6 * SLOT, K);
+  run_op<G, WPE>(sched::OP_MUL_PQ, sub, act, a, b, a, S, K);
+}
+
+// dst = src^|x| conjugated, as cyc_exp_grp, with the compact product
+template <int G, int WPE>
+__device__ __noinline__ void cyc_exp_compact(int sub, bool act, int src, int dst, int S, int K) {
+  run_op<G, WPE>(sched::OP_COPY, sub, act, src, 0, dst, S, K);
+  FirstRec<G, sched::OP_CYC_SQR> sq0;
+  sq0.load(sub);
+  for (int i = 62; i >= 0; i--) {
+    run_op_c<G, sched::OP_CYC_SQR>(sq0, sub, act, dst, 0, dst, S, K);
+    if ((BLS_X >> i) & 1) mul_compact<G>(sub, act, dst, src, S, K);
+  }
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, dst, 0, dst, S, K);
+}
+
+template <int G, int WPE>
+__device__ __forceinline__ void final_exp_tile_compact(int tile, const i32* __restrict__ fin, const i32* __restrict__ status,
+                                                       const i32* __restrict__ sub_status, int32_t* __restrict__ result,
+                                                       u64* __restrict__ gt_out, i32* __restrict__ park, int M) {
+  constexpr int IPW = Cfg<G>::IPW, L = Cfg<G>::L;
+  static_assert(G == L && G == 12, "one wavefront per item group, a lane per Fp12 component");
+  const int ln = threadIdx.x & 63, lid = ln / L, sub = ln % L;
+  const bool in_wg = lid < IPW;
+  const int item = tile * IPW + lid;
+  const bool valid = in_wg && item < M;
+  const int it = valid ? item : 0;
+  const int base = (in_wg ? lid : 0) * FEC_SLOTS * SLOT;
+  const int X = base + FEC_X, Y = base + FEC_Y, S = base + FEC_S;
+  const int K = IPW * FEC_SLOTS * SLOT;
+  const int FLAGS = K + sched::NUM_CONSTS * SLOT + (in_wg ? lid : 0) * 12;
+  // this lane's word of home h: the item's slot in the grid, then component sub
+  i32* const home = park + ((size_t)blockIdx.x * IPW + (in_wg ? lid : 0)) * FEC_HOMES * FEC_HOME_WORDS + sub * NL;
+  auto store_home = [&](int area, int h) {
+    if (in_wg) {
+#pragma unroll
+      for (int l = 0; l < NL; l++) home[h * FEC_HOME_WORDS + l] = g_lds[area + sub * SLOT + l];
+    }
+  };
+  auto load_home = [&](int h, int area) {
+    if (in_wg) {
+#pragma unroll
+      for (int l = 0; l < NL; l++) g_lds[area + sub * SLOT + l] = home[h * FEC_HOME_WORDS + l];
+    }
+    __syncthreads();
+  };
+  int32_t st = DEC_OK;
+  if (valid && status) st = status[it];
+  if (valid && sub_status && st == DEC_OK) st = sub_status[it];
+  const bool act = valid && (st == DEC_OK || st == DEC_IDENTITY);
+  if (in_wg) {
+    for (int w = sub; w < 12 * NL; w += G) {
+      const int e = w / NL, l = w % NL;
+      g_lds[X + e * SLOT + l] = act ? fin[(size_t)it * (12 * NL) + w] : 0;
+    }
+  }
+  __syncthreads();
+
+  // easy part (as final_exp_tile): f in X; the norm chain in Y and X; f waits at home
+  run_op<G, WPE>(sched::OP_N12, sub, act, X, 0, Y, S, K);                 // Y[0..5] = c0^2 - v c1^2
+  store_home(X, H_F);
+  run_op<G, WPE>(sched::OP_N6A, sub, act, Y, 0, Y + 6 * SLOT, S, K);     // Y[6..11] = (t0, t1, t2)
+  run_op<G, WPE>(sched::OP_N6B, sub, act, Y, Y + 6 * SLOT, X, S, K);     // X[0..1] = sc
+  run_op<G, WPE>(sched::OP_N2, sub, act, X, 0, X + 2 * SLOT, S, K);      // X[2] = n
+  // one inversion per workgroup, as in final_exp_tile, on X[2] (an inactive item's X
+  // holds the zeros it was loaded with: its n counts as 1 and gets a zero inverse)
+  {
+    const Fp n_own = lds_ld(X + 2 * SLOT);
+    const bool z = is_zero(n_own);
+    __syncthreads();
+    if (in_wg && sub == 0 && z) lds_st(X + 2 * SLOT, fp_one());
+    __syncthreads();
+    const bool pre = (threadIdx.x & 63) < 32;
+    Fp acc = fp_one();
+    for (int k = 0; k < IPW; k++) {
+      const int bq = (pre ? k : IPW - 1 - k) * FEC_SLOTS * SLOT + FEC_X;
+      if ((threadIdx.x & 31) == 0 && threadIdx.x < 64) lds_st(bq + (pre ? 6 : 7) * SLOT, acc);
+      acc = mul(acc, lds_ld(bq + 2 * SLOT));
+    }
+    const Fp inv_all = inv_fp_uniform(acc);
+    __syncthreads();
+    const Fp inv_own = mul(mul(inv_all, lds_ld(X + 6 * SLOT)), lds_ld(X + 7 * SLOT));
+    if (in_wg && sub == 0) lds_st(X + 3 * SLOT, z ? fp_zero() : inv_own);
+  }
+  __syncthreads();
+  run_op<G, WPE>(sched::OP_INV2, sub, act, X, X + 3 * SLOT, X + 4 * SLOT, S, K);  // X[4..5] = sc^-1
+  run_op<G, WPE>(sched::OP_INV6, sub, act, Y + 6 * SLOT, X + 4 * SLOT, Y, S, K);  // Y[0..5] = t^-1
+  load_home(H_F, X);
+  run_op<G, WPE>(sched::OP_INV12, sub, act, X, Y, X, S, K);                      // X = f^-1
+  load_home(H_F, Y);
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, Y, 0, Y, S, K);
+  mul_compact_i<G, WPE>(sub, act, Y, X, S, K);                                          // Y = f^(p^6-1)
+  run_op<G, WPE>(sched::OP_FROB2, sub, act, Y, 0, X, S, K);
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = t2
+
+  // hard part (api.cu:68-77), the same sequence as final_exp_tile with the values
+  // f t0 t1 t2 t3 t4 moved between X, Y and their homes
+  store_home(X, H_F);                                                            // f = t2
+  run_op<G, WPE>(sched::OP_CYC_SQR, sub, act, X, 0, Y, S, K);
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, Y, 0, Y, S, K);                       // Y = t1
+  store_home(Y, H_T1);
+  cyc_exp_compact<G, WPE>(sub, act, X, Y, S, K);                                 // Y = t3 = t2^x
+  store_home(X, H_T2);
+  load_home(H_T1, X);
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = t4 = t1 t3
+  store_home(Y, H_T3);
+  cyc_exp_compact<G, WPE>(sub, act, X, Y, S, K);                                 // Y = t1 = t4^x
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, X, 0, X, S, K);                       // X = conj(t4)
+  store_home(Y, H_T1);
+  load_home(H_F, Y);
+  mul_compact_i<G, WPE>(sub, act, Y, X, S, K);                                          // Y = f *= t4
+  store_home(Y, H_F);
+  load_home(H_T3, X);
+  run_op<G, WPE>(sched::OP_CYC_SQR, sub, act, X, 0, Y, S, K);                    // Y = t4 = cyc_sqr(t3)
+  store_home(Y, H_T4);
+  load_home(H_T1, X);
+  cyc_exp_compact<G, WPE>(sub, act, X, Y, S, K);                                 // Y = t0 = t1^x
+  load_home(H_T3, X);
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = t3 *= t0
+  run_op<G, WPE>(sched::OP_FROB2, sub, act, X, 0, X, S, K);                      // X = t3 = frob^2(t3)
+  store_home(Y, H_T0);
+  load_home(H_F, Y);
+  mul_compact_i<G, WPE>(sub, act, Y, X, S, K);                                          // Y = f *= t3
+  store_home(Y, H_F);
+  load_home(H_T0, X);
+  cyc_exp_compact<G, WPE>(sub, act, X, Y, S, K);                                 // Y = tmp = t0^x
+  load_home(H_T4, X);
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = t4 *= tmp
+  cyc_exp_compact<G, WPE>(sub, act, X, Y, S, K);                                 // Y = tmp = t4^x
+  store_home(X, H_T4);
+  load_home(H_F, X);
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = f *= tmp
+  store_home(X, H_F);
+  load_home(H_T4, X);
+  load_home(H_T2, Y);
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, Y, 0, Y, S, K);                       // Y = conj(t2)
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = t4 *= conj(t2)
+  run_op<G, WPE>(sched::OP_CONJ, sub, act, Y, 0, Y, S, K);                       // Y = t2
+  store_home(X, H_T4);
+  load_home(H_T1, X);
+  mul_compact_i<G, WPE>(sub, act, Y, X, S, K);                                          // Y = t2 *= t1
+  run_op<G, WPE>(sched::OP_FROB3, sub, act, Y, 0, Y, S, K);                      // Y = t2 = frob^3(t2)
+  load_home(H_F, X);
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = f *= t2
+  load_home(H_T4, Y);
+  run_op<G, WPE>(sched::OP_FROB1, sub, act, Y, 0, Y, S, K);                      // Y = t4 = frob(t4)
+  mul_compact_i<G, WPE>(sub, act, X, Y, S, K);                                          // X = f *= t4
+
+  // identity test and export, as final_exp_tile
+  if (in_wg) {
+    for (int c = sub; c < 12; c += G) {
+      int ok = 1;
+      if (act) {
+        const Fp v = canon(lds_ld(X + c * SLOT));
+        const Fp want = c == 0 ? canon(fp_one()) : fp_zero();
+        for (int l = 0; l < NL; l++) ok &= v.l[l] == want.l[l];
+        if (gt_out) st_ref(gt_out + (size_t)it * 72 + c * 6, lds_ld(X + c * SLOT));
+      } else if (valid && gt_out) {
+        for (int j = 0; j < 6; j++) gt_out[(size_t)it * 72 + c * 6 + j] = 0;
+      }
+      g_lds[FLAGS + c] = ok;
+    }
+  }
+  __syncthreads();
+  if (valid && sub == 0) {
+    int all = 1;
+    for (int k = 0; k < 12; k++) all &= g_lds[FLAGS + k];
+    result[it] = act ? all : st;
+  }
+  __syncthreads();  // the next tile reuses this tile's LDS
+}
+
+// queue-fed persistent grid only (its homes are indexed by workgroup), two waves per SIMD
+template <int G>
+__global__ __launch_bounds__(Cfg<G>::THREADS, 2) void k_final_exp_compact(const i32* __restrict__ fin, const i32* __restrict__ status,
+                                                           const i32* __restrict__ sub_status, int32_t* __restrict__ result,
+                                                           u64* __restrict__ gt_out, int M, int* queue, i32* __restrict__ park) {
+  constexpr int IPW = Cfg<G>::IPW;
+  const int ntiles = (M + IPW - 1) / IPW;
+  const int K = IPW * FEC_SLOTS * SLOT;
+  __builtin_amdgcn_s_setprio(2);
+  load_consts(K);
+  for (int round = 0;; round++) {
+    const int tile = next_tile(queue, round);
+    if (tile >= ntiles) break;
+    final_exp_tile_compact<G, 2>(tile, fin, status, sub_status, result, gt_out, park, M);
+  }
+}
+#endif  // GRP_KIND == 2
+
 // ---------------- launch wrappers ----------------
 // Resident workgroups per CU of each kernel instantiation, queried once (callers hold
 // the library lock) so that no occupancy query happens during graph capture.
@@ -849,6 +1073,11 @@ int grp_occ_fe() {
   static int occ = 0;
   if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_grp<G, false>, Cfg<G>::THREADS, final_exp_lds_bytes<G>()) != hipSuccess || occ <= 0)) occ = 1;
   return occ;
+}
+template <int G>
+int grp_occ_fe_compact() {
+  if constexpr (G == 12) return fe_compact_occ();
+  else return 0;
 }
 // Grid of a queue-fed launch: at most wg_per_cu workgroups per CU (0 = as many as
 // fit), never more than the occupancy limit, never more than there are tiles.
@@ -957,12 +1186,23 @@ int grp_launch_ml(const int32_t* sig, const int32_t* nhm, const int32_t* gen, co
 }
 template <int G>
 int grp_launch_fe(const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result, uint64_t* gt,
-                  int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, hipStream_t s) {
+                  int M, int* queue, int wg_per_cu, const SubgroupRiders& riders, int32_t* park, long park_items,
+                  hipStream_t s) {
   constexpr int IPW = Cfg<G>::IPW;
   const int ntiles = (M + IPW - 1) / IPW;
   int grid = 0;
   if (riders.per && !rides<G>(1)) return -1;
   const int occ = queue ? grp_occ_fe<G>() : 0;
+  // more workgroups per CU than the full-LDS kernel holds: the compact one, whose homes
+  // are indexed by workgroup, so only on a queue-fed grid whose items all have homes
+  if constexpr (G == 12) {
+    const int occ_c = queue ? fe_compact_occ() : 0;
+    if (queue && park && wg_per_cu > occ && occ_c > occ) {
+      grid = queue_grid(occ_c, ntiles, wg_per_cu);
+      if ((long)grid * IPW <= park_items)
+        return fe_compact_launch(grid, fin, status, sub_status, result, gt, M, queue, park, s);
+    }
+  }
   if (launch_grid(occ, ntiles, Cfg<G>::THREADS, queue, wg_per_cu, riders, M, &grid)) return -1;
   if constexpr (rides<G>(1)) {
     if (riders.per || at_most_one_wave_per_simd(queue, occ, wg_per_cu, Cfg<G>::W)) {
@@ -988,8 +1228,25 @@ template int grp_occ_ml<GRP_LANES>();
 #endif
 #if GRP_KIND == 1 && (0 BLS_GRP_FE_LANES(BLS_IS))
 template int grp_launch_fe<GRP_LANES>(const int32_t*, const int32_t*, const int32_t*, int32_t*, uint64_t*, int, int*, int,
-                                      const SubgroupRiders&, hipStream_t);
+                                      const SubgroupRiders&, int32_t*, long, hipStream_t);
 template int grp_occ_fe<GRP_LANES>();
+template int grp_occ_fe_compact<GRP_LANES>();
+#endif
+#if GRP_KIND == 2
+int fe_compact_occ() {
+  static int occ = -1;
+  if (occ < 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_final_exp_compact<12>, Cfg<12>::THREADS,
+                                                               final_exp_compact_lds_bytes<12>()) != hipSuccess ||
+                  occ < 0))
+    occ = 0;
+  return occ;
+}
+int fe_compact_launch(int grid, const int32_t* fin, const int32_t* status, const int32_t* sub_status, int32_t* result,
+                      uint64_t* gt, int M, int* queue, int32_t* park, hipStream_t s) {
+  hipLaunchKernelGGL((k_final_exp_compact<12>), dim3(grid), dim3(Cfg<12>::THREADS), final_exp_compact_lds_bytes<12>(),
+                     s, fin, status, sub_status, result, gt, M, queue, park);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 #endif
 #undef BLS_IS
 

@@ -19,7 +19,8 @@ static int final_exp(const EngineArgs& a, const int32_t* status, const int32_t* 
     return launch_final_exp(a.fbuf, status, a.results, a.gt, a.M, a.block, s);
   }
   return launch_final_exp_grp(a.g_fe, a.fbuf, status, sub_status, a.results, a.gt, a.M,
-                              a.queue && a.wpc_fe >= 0 ? a.queue + 1 : nullptr, a.wpc_fe, riders, s);
+                              a.queue && a.wpc_fe >= 0 ? a.queue + 1 : nullptr, a.wpc_fe, riders, a.park,
+                              a.park_items, s);
 }
 
 int engine_legacy_setup(const EngineArgs& a, hipStream_t cs) {

@@ -41,6 +41,10 @@ struct EngineArgs {
   int* queue = nullptr;
   int wpc_ml = 0;
   int wpc_fe = 0;
+  // homes of the compact final exponentiation (final_exp_park_words() words) for park_items
+  // resident items; null: queue-fed launches keep the full-LDS kernel
+  int32_t* park = nullptr;
+  long park_items = 0;
   // key-line depth of the Miller loop (launch_miller_grp; -1 = the default)
   int depth_ml = -1;
   // optional timing: events recorded before each phase and at the end (5 events)

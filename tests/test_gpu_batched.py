@@ -87,8 +87,11 @@ def _call_device(bufs, off):
 
 
 @pytest.mark.parametrize("cfg", [(12, 16, 0, 0), (12, 16, 1, 1), (13, 18, 2, 1), (16, 16, -1, -1), (8, 32, 1, 2),
-                                 (32, 12, 0, 1)], ids=lambda c: "ml%d_fe%d_w%d_%d" % c)
+                                 (32, 12, 0, 1), (32, 12, 4, 8), (12, 12, 5, 6)],
+                         ids=lambda c: "ml%d_fe%d_w%d_%d" % c)
 def test_device_batches_queue_configs(cfg, large):
+    # 12-lane final exponentiations at more than four workgroups per CU run its compact
+    # form (two Fp12 values in LDS, the others in device memory)
     print("case: device batched form 1600 items, config", cfg, flush=True)
     h = _register(large)
     blsgpu.set_batched_engine(*cfg, True)

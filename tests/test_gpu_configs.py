@@ -116,7 +116,7 @@ def test_uniform_4096_legacy_abi(uniform, engine):
         blsgpu.set_engine(*saved)
 
 
-@pytest.mark.parametrize("cfg", [(12, 12, 0, 0), (32, 24, 0, 0)], ids=lambda c: "ml%d_fe%d_w%d_%d" % c)
+@pytest.mark.parametrize("cfg", [(12, 12, 0, 0), (32, 24, 0, 0), (32, 12, 4, 8)], ids=lambda c: "ml%d_fe%d_w%d_%d" % c)
 def test_at_scale_65536_batched_form(at_scale, cfg):
     print("case: at scale 65536 items / 487 batches / 40 keys, batched form", cfg, flush=True)
     w = at_scale

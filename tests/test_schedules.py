@@ -27,10 +27,11 @@ def decode(term):
     return (term >> 6) & 3, term & 63, coef
 
 
-def run_op(rounds, a, b):
-    """execute one op's rounds (as the kernel does) on input values a, b (lists of ints)"""
+def run_op(rounds, a, b, t=None):
+    """execute one op's rounds (as the kernel does) on input values a, b (lists of ints);
+    t: values of T slots that an earlier op filled (the op's reserved slots)"""
     prounds, orounds = rounds
-    regions = {A: a, B: b, K: [v % P for v in GSCH.CONSTS.vals], T: {}}
+    regions = {A: a, B: b, K: [v % P for v in GSCH.CONSTS.vals], T: dict(t or {})}
 
     def val(term):
         reg, idx, coef = decode(term)
@@ -60,7 +61,12 @@ def run_op(rounds, a, b):
     return [out.get(i) for i in range(max(out) + 1)]
 
 
-def symbolic(prods, outs, a, b):
+def reserved(prods, rnd):
+    """random values for the T slots an op reserves (filled by an earlier op)"""
+    return {i: rnd.randrange(P) for i, p in enumerate(prods) if p is None}
+
+
+def symbolic(prods, outs, a, b, t=None):
     vals = {A: a, B: b, K: [v % P for v in GSCH.CONSTS.vals]}
 
     def ev(lin, tv):
@@ -69,7 +75,7 @@ def symbolic(prods, outs, a, b):
             s += c * (tv[idx] if reg == T else vals[reg][idx])
         return s % P
 
-    tv = [ev(x, None) * ev(y, None) % P for x, y in prods]
+    tv = [t[i] if p is None else ev(p[0], None) * ev(p[1], None) % P for i, p in enumerate(prods)]
     return [None if o is None else ev(o, tv) for o in outs]
 
 
@@ -120,12 +126,14 @@ def test_round_tables_execute_every_op(variants, G):
             assert len(recs) == G
             busy = [r[0] != GSCH.IDLE for r in recs]
             assert busy == sorted(busy, reverse=True)
-        assert sum(1 for _, _, recs in prounds for r in recs if r[0] != GSCH.IDLE) == len(prods)
+        assert sum(1 for _, _, recs in prounds for r in recs if r[0] != GSCH.IDLE) == \
+            sum(p is not None for p in prods)
         assert sum(1 for _, _, recs in orounds for r in recs if r[0] != GSCH.IDLE) == sum(o is not None for o in outs)
         for _ in range(3):
             a = [rnd.randrange(P) for _ in range(12)]
             b = [rnd.randrange(P) for _ in range(28)]
-            assert run_op((prounds, orounds), a, b) == symbolic(prods, outs, a, b), name
+            t = reserved(prods, rnd)
+            assert run_op((prounds, orounds), a, b, t) == symbolic(prods, outs, a, b, t), name
 
 
 def tower(vals):
@@ -149,6 +157,21 @@ def test_fp12_mul_and_square_against_schoolbook(variants, G):
         assert run_op(mul, a, b) == flat(want)
         want2 = PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(a)), PF.f12_from_tower(tower(a))))
         assert run_op(sqr, a, [0] * 12) == flat(want2)
+
+
+@pytest.mark.parametrize("G", [12, 16, 32])
+def test_split_fp12_mul_against_schoolbook(variants, G):
+    """OP_MUL_S then OP_MUL_PQ (s passed in the first six T slots) equal the Fp12 product"""
+    rnd = random.Random(400 + G)
+    ops = {name: (p, o) for name, p, o, _, _ in GSCH.build_rounds_v(G, variants)}
+    for _ in range(4):
+        a = [rnd.randrange(P) for _ in range(12)]
+        b = [rnd.randrange(P) for _ in range(12)]
+        s = run_op(ops["OP_MUL_S"], a, b)
+        assert len(s) == 6
+        got = run_op(ops["OP_MUL_PQ"], a, b, dict(enumerate(s)))
+        want = PF.f12_to_tower(PF.f12mul_w(PF.f12_from_tower(tower(a)), PF.f12_from_tower(tower(b))))
+        assert got == flat(want)
 
 
 @pytest.mark.parametrize("G", [12, 16, 20, 32])
@@ -239,7 +262,8 @@ def test_static_tables_execute_the_straight_line_ops(variants, G):
         for _ in range(2):
             a = [rnd.randrange(P) for _ in range(12)]
             b = [rnd.randrange(P) for _ in range(28)]
-            regions = {A: a, B: b, T: {}}
+            t0 = reserved(prods, rnd)
+            regions = {A: a, B: b, T: dict(t0)}
 
             def term(t, reg):
                 if reg == GSCH.MIXED:
@@ -289,7 +313,7 @@ def test_static_tables_execute_the_straight_line_ops(variants, G):
                     else:
                         out[hw[0]] = sum(term(t, ra) for t in hw[1:1 + na]) % P
             got = [out.get(i) for i in range(max(out) + 1)]
-            assert got == symbolic(prods, outs, a, b), (G, name)
+            assert got == symbolic(prods, outs, a, b, t0), (G, name)
 
 
 def test_generated_header_is_current(variants):
